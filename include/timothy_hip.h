@@ -1,0 +1,210 @@
+/*
+ * timothy_hip.h -- native C ABI of the MI355X cell-mechanics path.
+ *
+ * One context per HIP device and stream.  Device-resident colonies (one, or
+ * many independent colonies batched in one set of launches) go through:
+ *
+ *   tm_upload / tm_bind_device      cells, structure of arrays
+ *   tm_build                        per-colony bounding box and affine map,
+ *                                   29-bit location codes, Morton sort,
+ *                                   PR-octree leaf octants, radix tree
+ *   tm_neighbors                    tree walk -> neighbour lists, mindist,
+ *                                   first-visited candidate (for v)
+ *   tm_sph                          density, v, forces: TM_FP64 (bit-exact
+ *                                   drop-in) or TM_MIXED
+ *   tm_cic_deposit / tm_field_gradient / tm_cic_interp
+ *                                   cells <-> grid transfers
+ *
+ * Plain pointers and sizes only.  Every call returns a tm_status; TM_OK is 0.
+ * Calls are stream-ordered on the context's stream; a context is not
+ * thread-safe.  Result getters synchronise the stream.
+ *
+ * Reference interfaces these entry points replace (mcytu/timothy):
+ *   tm_build          octbuild                        src/octree.c:145-207
+ *   tm_neighbors      the tree walks of computepotential / computegradient
+ *                     (octcomputebox, octlocateregion, octheap*, DFS)
+ *                                                     src/potential.c:87-138,209-259
+ *   tm_sph            computepotential + computegradient pair sums
+ *                                                     src/potential.c:39-259, src/inline.h:40-83
+ *   tm_cic_deposit    patches_cells2envinfo + patches_applycells2env
+ *                                                     src/patches.c:156-234,321-367
+ *   tm_cic_interp     patches_env2cellsinfo + patches_applyenv2cells
+ *                                                     src/patches.c:380-439,512-593
+ *   tm_field_gradient fieldgradient_compute            src/fieldgradient.c:149-224
+ *   tm_get_octree     the octnode_t array octbuild leaves in cellsinfo->octree
+ *                                                     src/global.h:153-164
+ * The reference's own signatures are exported by libtimothy_compat
+ * (timothy_compat.h).
+ */
+#ifndef TIMOTHY_HIP_H
+#define TIMOTHY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tm_status {
+  TM_OK = 0,
+  TM_ERR_INVALID = 1,     /* bad argument or size */
+  TM_ERR_HIP = 2,         /* HIP runtime error */
+  TM_ERR_NOMEM = 3,       /* device or host allocation failed */
+  TM_ERR_DEGENERATE = 4,  /* two cells share location codes down to bit 1 */
+  TM_ERR_STATE = 5,       /* call out of order (e.g. tm_sph before tm_neighbors) */
+  TM_ERR_UNSUPPORTED = 6  /* e.g. more than one MPI rank in the compat layer */
+} tm_status;
+
+typedef enum tm_precision {
+  TM_FP64 = 0,  /* reference arithmetic, bit-exact with the CPU path */
+  TM_MIXED = 1  /* FP32 pair evaluation, FP64 predicate and FP64 per-cell sums */
+} tm_precision;
+
+typedef enum tm_stage {
+  TM_STAGE_BUILD = 0,
+  TM_STAGE_NEIGHBORS = 1,
+  TM_STAGE_SPH = 2,
+  TM_STAGE_CIC = 3,
+  TM_STAGE_STEP = 4
+} tm_stage;
+
+/* mechanics fields of celltype_t (src/global.h:315-335), as given:
+ * h2..h4 are NOT recomputed from h (the reference reads them as stored). */
+typedef struct tm_celltype {
+  double size, h, h2, h3, h4;
+} tm_celltype;
+
+/* grid_t restricted to one rank (src/global.h:182-190, src/grid.c:37-136):
+ * node (i,j,k) of the local block sits at lower + (loweridx + (i,j,k))*resolution,
+ * arrays are [i][j][k], k fastest; gradients are 3-interleaved. */
+typedef struct tm_grid {
+  double lower[3];
+  float resolution;
+  int64_t loweridx[3];
+  int64_t upperidx[3];
+  int64_t localsize[3];
+} tm_grid;
+
+typedef struct tm_ctx tm_ctx;
+
+/* -------- context -------- */
+tm_status tm_create(tm_ctx **ctx, int device);
+void tm_destroy(tm_ctx *ctx);
+const char *tm_status_name(tm_status s);
+/* last error message of this context ("" if none) */
+const char *tm_last_error(const tm_ctx *ctx);
+/* the context's HIP stream (hipStream_t) */
+void *tm_stream(tm_ctx *ctx);
+tm_status tm_sync(tm_ctx *ctx);
+
+/* -------- configuration -------- */
+/* dimension 2 or 3 (src/global.h:177); ntypes <= 16 */
+tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes, int32_t dimension);
+/* tunables: "nbr_cap" neighbour slots per cell (grows on overflow),
+ * "sph_block" / "nbr_block" workgroup sizes, "sph_lanes" lanes per cell of the
+ * mixed kernel (1 = one lane per cell), "mixed_recentre" (1 = FP32 positions
+ * re-centred per 64-cell block, 0 = FP64 positions) */
+tm_status tm_set_param(tm_ctx *ctx, const char *key, int64_t value);
+tm_status tm_get_param(tm_ctx *ctx, const char *key, int64_t *value);
+
+/* -------- cells -------- */
+/* Host arrays (reference order).  ncolonies >= 1 independent colonies are
+ * concatenated; colony_offsets has ncolonies+1 entries (NULL when
+ * ncolonies == 1).  phase may be NULL (treated as 1), young may be NULL
+ * (treated as 0, like the reference's uninitialised file cells). */
+tm_status tm_upload(tm_ctx *ctx, int64_t ncells, int32_t ncolonies, const int64_t *colony_offsets,
+                    const double *x, const double *y, const double *z, const double *size,
+                    const float *young, const int32_t *ctype, const int32_t *phase);
+/* The same with device pointers already in HBM (no copy; they must stay valid
+ * and unchanged until the next tm_upload/tm_bind_device). colony_offsets is a
+ * HOST array. */
+tm_status tm_bind_device(tm_ctx *ctx, int64_t ncells, int32_t ncolonies, const int64_t *colony_offsets,
+                         const double *x, const double *y, const double *z, const double *size,
+                         const float *young, const int32_t *ctype, const int32_t *phase);
+
+/* -------- path stages -------- */
+tm_status tm_build(tm_ctx *ctx);
+tm_status tm_neighbors(tm_ctx *ctx);
+tm_status tm_sph(tm_ctx *ctx, tm_precision precision);
+/* build + neighbors + sph */
+tm_status tm_step(tm_ctx *ctx, tm_precision precision);
+
+/* -------- results (host, reference cell order) -------- */
+/* any pointer may be NULL */
+tm_status tm_get_results(tm_ctx *ctx, double *density, double *v, double *mindist, double *fx, double *fy,
+                         double *fz);
+/* results in Morton-sorted order plus the permutation (sorted -> reference
+ * index), device pointers valid until the next stage call; the layout the
+ * stages keep in HBM */
+tm_status tm_device_results(tm_ctx *ctx, const double **density, const double **v, const double **mindist,
+                            const double **fx, const double **fy, const double **fz, const int32_t **perm);
+tm_status tm_get_affine(tm_ctx *ctx, int32_t colony, double shift[3], double *scale);
+/* 3 location codes per cell (x,y,z), reference order */
+tm_status tm_get_codes(tm_ctx *ctx, uint32_t *loc);
+/* perm[i] = reference index of the i-th cell in Morton order; span_log2[c] =
+ * log2 of the side (in code units) of cell c's PR-octree leaf octant */
+tm_status tm_get_order(tm_ctx *ctx, int32_t *perm, int32_t *span_log2);
+/* in-cutoff neighbour lists in the reference's accumulation order (descending
+ * Morton key), reference indices; offsets has ncells+1 entries; nbr may be
+ * NULL to get the total only */
+tm_status tm_get_neighbors(tm_ctx *ctx, int64_t *offsets, int32_t *nbr, int64_t *total);
+/* first-visited-last candidate (minimum Morton key) per cell, -1 if none */
+tm_status tm_get_qstar(tm_ctx *ctx, int32_t *qstar);
+/* P = ordered in-cutoff pairs; max_nbr = largest neighbour count */
+tm_status tm_pair_count(tm_ctx *ctx, int64_t *pairs, int64_t *max_nbr);
+/* canonical PR-octree of one colony as octnode_t records (112 bytes each,
+ * src/global.h:153-164): internal nodes first (root = 0), then one leaf per
+ * cell; data holds the reference cell index, -2 for internal nodes.  Pass
+ * nodes == NULL to get the count. */
+tm_status tm_get_octree(tm_ctx *ctx, int32_t colony, void *nodes, int64_t *count);
+
+/* -------- grid transfers (single colony; device arrays) -------- */
+/* weight_mode 0: reference weights by phase (5 -> 0, 0 -> 0.75, else 1.0);
+ * weight_mode 1: cell volume 4/3*pi*size^3.  production is overwritten
+ * (zeroed first, like patches_applycells2env). */
+tm_status tm_cic_deposit(tm_ctx *ctx, const tm_grid *grid, int32_t weight_mode, double *production);
+/* gradient of a field with the reference's stencil (interior (f[i+1]-f[i-1])/res,
+ * one-sided faces); FP64 or FP32 field */
+tm_status tm_field_gradient(tm_ctx *ctx, const tm_grid *grid, const double *field, double *gradient);
+tm_status tm_field_gradient_f32(tm_ctx *ctx, const tm_grid *grid, const float *field, float *gradient);
+/* out[4*c + {0,1,2,3}] = value, gx, gy, gz at cell c (reference order) */
+tm_status tm_cic_interp(tm_ctx *ctx, const tm_grid *grid, const double *field, const double *gradient,
+                        double *out);
+tm_status tm_cic_interp_f32(tm_ctx *ctx, const tm_grid *grid, const float *field, const float *gradient,
+                            double *out);
+
+/* -------- device memory for callers without their own allocator -------- */
+tm_status tm_malloc(tm_ctx *ctx, void **ptr, size_t bytes);
+tm_status tm_free(tm_ctx *ctx, void *ptr);
+tm_status tm_memcpy_htod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
+tm_status tm_memcpy_dtoh(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
+tm_status tm_memset(tm_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* -------- measurement -------- */
+/* runs `warmup` untimed then `iters` timed launches of a stage on the
+ * context's stream, bracketed by HIP events on that stream; ms_per_iter is
+ * the event time / iters.  For TM_STAGE_CIC the grid/production of the last
+ * tm_cic_deposit call are reused. */
+tm_status tm_time_stage(tm_ctx *ctx, tm_stage stage, tm_precision precision, int32_t warmup, int32_t iters,
+                        double *ms_per_iter);
+/* per-kernel average duration (ms) of the SPH kernel over the last
+ * tm_time_stage(TM_STAGE_SPH) call */
+tm_status tm_last_kernel_ms(tm_ctx *ctx, double *ms);
+
+/* -------- captured step -------- */
+/* capture build + neighbors + sph into a hipGraph (no allocation inside);
+ * tm_graph_launch replays it and checks the overflow / degeneracy flags */
+tm_status tm_graph_capture(tm_ctx *ctx, tm_precision precision);
+tm_status tm_graph_launch(tm_ctx *ctx);
+tm_status tm_graph_destroy(tm_ctx *ctx);
+
+/* -------- autotuner -------- */
+/* times candidate launch shapes for the current colonies and keeps the fastest;
+ * writes a short report into buf (may be NULL) */
+tm_status tm_autotune(tm_ctx *ctx, tm_precision precision, char *buf, size_t buflen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,287 @@
+"""GPU parity: the HIP path through its C ABI against the CPU oracle.
+
+Bars: location codes, affine map, Morton order, leaf octants, neighbour
+lists (content and order), first-visited candidate, CIC indices, field
+gradient and interpolation are bit-exact; FP64 density, v, mindist and forces
+are bit-exact (the tests also state the 1e-12 relative bound the drop-in
+promises); the CIC deposit sums the same terms with atomics, within 1e-12
+relative; the mixed form is within 1e-5 of FP64 (force error normalised per
+cell by sum_q |f_pq|).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from timothy_amd import generators as gen
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx(tm):
+    c = tm.Context(0)
+    yield c
+    c.close()
+
+
+def run_both(tm, O, ctx, col, precision=0):
+    say(f"case n={col['n']} precision={precision}")
+    ctx.upload_colony(col)
+    ctx.build()
+    ctx.neighbors()
+    ctx.sph(precision)
+    res = ctx.results()
+    oc = O.Colony.from_dict(col)
+    tree = O.Octree(oc, col["types"])
+    return res, oc, tree
+
+
+def oracle_results(tree):
+    dens, v, md = tree.computepotential()
+    F = tree.computegradient()
+    return dict(density=dens, v=v, mindist=md, force=F)
+
+
+CASES = [
+    ("ball20k", lambda: gen.config(0)),
+    ("gauss2types", lambda: gen.gaussian_colony(30000, 77, 146.6 * (30000 / 2e6) ** (1 / 3), frac_type1=0.05)),
+    ("gauss_c1_small", lambda: gen.config(1, scale=0.02)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES)
+def test_build_and_neighbors_exact(tm, oracle_mod, ctx, name, make):
+    O = oracle_mod
+    col = make()
+    say(f"build/neighbours {name} n={col['n']}")
+    ctx.upload_colony(col)
+    ctx.build()
+    oc = O.Colony.from_dict(col)
+    tree = O.Octree(oc, col["types"])
+    sh, sc = ctx.affine()
+    assert np.array_equal(sh, tree.shift) and sc == tree.scale
+    assert np.array_equal(ctx.codes(), tree.loc())
+    perm, span = ctx.order()
+    assert np.array_equal(span, tree.leaf_levels() + 1)
+    keys = np.array([H.morton_key(l) for l in tree.loc()], dtype=object)
+    ks = keys[perm]
+    assert all(ks[i] < ks[i + 1] for i in range(len(ks) - 1))
+    ctx.neighbors()
+    off, nbr = ctx.neighbor_lists()
+    coff, cand, inh = tree.candidates()
+    q = ctx.qstar()
+    for p in range(col["n"]):
+        c = cand[coff[p]:coff[p + 1]]
+        exp = c[inh[coff[p]:coff[p + 1]] == 1]
+        assert np.array_equal(nbr[off[p]:off[p + 1]], exp), p
+        assert q[p] == (c[-1] if len(c) else -1), p
+    P, mx = ctx.pair_count()
+    assert P == int(inh.sum())
+    say(f"  pairs={P} candidates={len(cand)} max_nbr={mx}")
+
+
+@pytest.mark.parametrize("name,make", CASES)
+def test_sph_fp64_bit_exact(tm, oracle_mod, ctx, name, make):
+    col = make()
+    res, oc, tree = run_both(tm, oracle_mod, ctx, col, tm.FP64)
+    ref = oracle_results(tree)
+    for k in ("density", "v", "mindist", "force"):
+        a, b = res[k], ref[k]
+        scale = np.maximum(np.abs(b), 1e-300)
+        assert np.all(np.abs(a - b) <= 1e-12 * scale), k
+        assert np.array_equal(a, b), k
+
+
+def _pair_abs_force(col, off, nbr):
+    """sum_q |f_pq| = sum_q |w(u)| * 48/(pi h4) per cell, from the neighbour lists"""
+    n = col["n"]
+    types = col["types"]
+    P = np.stack([col["x"], col["y"], col["z"]], 1)
+    cnt = np.diff(off)
+    rows = np.repeat(np.arange(n), cnt)
+    d = np.sqrt(((P[rows] - P[nbr]) ** 2).sum(1))
+    h = np.array([types[t]["h"] for t in col["ctype"]])[rows]
+    h4 = np.array([types[t]["h4"] for t in col["ctype"]])[rows]
+    u = d / h
+    w = np.where(u <= 0.5, -2 * u + 3 * u * u, -(1 - u) ** 2)
+    s = np.zeros(n)
+    np.add.at(s, rows, np.abs(w) * 48 / (np.pi * h4))
+    return s
+
+
+@pytest.mark.parametrize("name,make", CASES)
+def test_sph_mixed_error(tm, oracle_mod, ctx, name, make):
+    col = make()
+    res, oc, tree = run_both(tm, oracle_mod, ctx, col, tm.MIXED)
+    ref = oracle_results(tree)
+    off, nbr = ctx.neighbor_lists()
+    sabs = _pair_abs_force(col, off, nbr)
+    has = sabs > 0
+    ferr = np.abs(res["force"] - ref["force"]).max(axis=1)[has] / sabs[has]
+    dens_err = np.abs(res["density"] - ref["density"])[ref["density"] > 0] / ref["density"][ref["density"] > 0]
+    say(f"  mixed {name}: force max {ferr.max():.3e} rms {np.sqrt((ferr ** 2).mean()):.3e}; "
+        f"density max {dens_err.max():.3e}")
+    assert ferr.max() <= 1e-5
+    assert dens_err.max() <= 1e-5
+    assert np.array_equal(res["mindist"], ref["mindist"])
+    vs = np.abs(ref["v"]).max()
+    assert np.abs(res["v"] - ref["v"]).max() <= 1e-4 * vs
+
+
+def test_vtk_fixture_on_gpu(tm, oracle_mod, ctx):
+    O = oracle_mod
+    fx = json.load(open(os.path.join(GOLD, "vtk_step0.json")))
+    P = np.array(fx["positions_after_move_f32"], np.float64)
+    F = np.array(fx["force_f32"], np.float64)
+    X = P - F
+    n = fx["n"]
+    col = dict(n=n, x=X[:, 0].copy(), y=X[:, 1].copy(), z=X[:, 2].copy(), size=np.full(n, 5.0),
+               young=np.full(n, 2100.0, np.float32), ctype=np.zeros(n, np.int32), phase=np.ones(n, np.int32),
+               types=[fx["celltype"]])
+    res, oc, tree = run_both(tm, O, ctx, col, tm.FP64)
+    assert np.abs(res["force"] - F).max() / np.abs(F).max() < 1e-5
+    assert np.array_equal(res["force"], tree.computegradient())
+
+
+def test_small_sizes_and_errors(tm, oracle_mod, ctx):
+    types = gen.CELLTYPES
+    # one cell: nothing to sum; outputs of the drop-in would be left untouched
+    ctx.set_celltypes(types)
+    ctx.upload([1.0], [2.0], [3.0], [1.0], np.array([2100.0], np.float32), [0])
+    ctx.build()
+    ctx.neighbors()
+    assert ctx.pair_count() == (0, 0)
+    # two touching cells
+    ctx.upload([0.0, 1.5], [0.0, 0.0], [0.0, 0.0], [1.0, 1.0], np.array([2100.0, 2000.0], np.float32), [0, 0])
+    ctx.step(tm.FP64)
+    r = ctx.results()
+    oc = oracle_mod.Colony([0.0, 1.5], [0.0, 0.0], [0.0, 0.0], [1.0, 1.0], np.array([2100.0, 2000.0], np.float32),
+                           [0, 0])
+    ref = oracle_results(oracle_mod.Octree(oc, types))
+    for k in ref:
+        assert np.array_equal(r[k], ref[k]), k
+    # duplicate positions: the reference recurses forever; here an error
+    ctx.upload([0.0, 0.0, 3.0], [0.0, 0.0, 3.0], [0.0, 0.0, 3.0], [1.0] * 3, np.full(3, 2100.0, np.float32), [0] * 3)
+    with pytest.raises(tm.TMError) as e:
+        ctx.build()
+    assert e.value.status == 4
+    # bad cell type
+    with pytest.raises(tm.TMError):
+        ctx.upload([0.0], [0.0], [0.0], [1.0], None, [7])
+
+
+def test_octree_node_set(tm, oracle_mod, ctx):
+    O = oracle_mod
+    col = gen.gaussian_colony(5000, 5, 146.6 * (5000 / 2e6) ** (1 / 3), frac_type1=0.1)
+    ctx.upload_colony(col)
+    ctx.build()
+    g = ctx.octree()
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    o = tree.nodes()
+    assert len(g) == len(o)
+    key = lambda a: sorted(zip(a["level"].tolist(), map(tuple, a["code"].tolist()), map(tuple, a["limit"].tolist()),
+                               a["data"].tolist()))
+    assert key(g) == key(o)
+    assert g["father"][0] == -1 and g["level"][0] == 29
+    # every child link points back to its father, every node but the root has one
+    for i in range(len(g)):
+        for ch in g["child"][i]:
+            if ch >= 0:
+                assert g["father"][ch] == i
+    assert np.all(g["father"][1:] >= 0)
+    # locating a query box from the emitted array agrees with the oracle's own tree
+    n_int = int((g["data"] == -2).sum())
+    assert np.all(g["data"][:n_int] == -2) and np.all(g["data"][n_int:] >= 0)
+
+
+def test_cic_and_field_gradient(tm, oracle_mod, ctx):
+    O = oracle_mod
+    col = gen.config(0)
+    grid = col["grid"]
+    G = grid["G"]
+    ctx.upload_colony(col)
+    ctx.build()
+    oc = O.Colony.from_dict(col)
+    for mode in (0, 1):
+        prod = tm.DeviceArray(ctx, G ** 3, np.float64)
+        ctx.cic_deposit(grid, prod, mode)
+        got = prod.get()
+        ref = O.cic_deposit(oc, grid, mode)
+        assert np.array_equal(got != 0, ref != 0)
+        assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref))
+        say(f"  deposit mode {mode}: max rel {np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)):.2e}")
+    field = gen.smooth_field(grid, 5, np.float64)
+    f_d = tm.DeviceArray.from_host(ctx, field)
+    g_d = tm.DeviceArray(ctx, G ** 3 * 3, np.float64)
+    ctx.field_gradient(grid, f_d, g_d)
+    grad = g_d.get()
+    gref = O.fieldgradient(grid["localsize"], grid["resolution"], field)
+    assert np.array_equal(grad, gref)
+    out = tm.DeviceArray(ctx, col["n"] * 4, np.float64)
+    ctx.cic_interp(grid, f_d, g_d, out)
+    iref = O.cic_interp(oc, grid, field, gref)
+    assert np.array_equal(out.get().reshape(-1, 4), iref)
+
+
+def test_batched_colonies_match_single(tm, oracle_mod, ctx):
+    O = oracle_mod
+    cols = gen.colonies_config3(ncol=12, scale=0.05)
+    say(f"batched colonies n={[c['n'] for c in cols]}")
+    off = ctx.upload_colonies(cols)
+    ctx.step(tm.FP64)
+    res = ctx.results()
+    for i, c in enumerate(cols):
+        sl = slice(off[i], off[i + 1])
+        ref = oracle_results(O.Octree(O.Colony.from_dict(c), c["types"]))
+        for k in ("density", "v", "mindist", "force"):
+            assert np.array_equal(res[k][sl], ref[k]), (i, k)
+        sh, sc = ctx.affine(i)
+        t = O.Octree(O.Colony.from_dict(c), c["types"])
+        assert np.array_equal(sh, t.shift) and sc == t.scale
+
+
+def test_graph_replay_matches_eager(tm, ctx):
+    col = gen.config(1, scale=0.02)
+    ctx.upload_colony(col)
+    ctx.step(tm.MIXED)
+    eager = ctx.results()
+    ctx.graph_capture(tm.MIXED)
+    for rep in range(3):
+        ctx.graph_launch()
+        r = ctx.results()
+        for k in eager:
+            assert np.array_equal(r[k], eager[k]), (rep, k)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_full_size_properties(tm, ctx, precision):
+    """2M-cell Gaussian spheroid at its full size: properties that do not need
+    the CPU path -- neighbour symmetry, Newton's third law, deposit mass."""
+    col = gen.config(1)
+    say(f"full size n={col['n']} precision={precision}")
+    ctx.upload_colony(col)
+    ctx.step(precision)
+    P, mx = ctx.pair_count()
+    say(f"  pairs={P} max_nbr={mx}")
+    res = ctx.results()
+    F = res["force"]
+    scale = np.abs(F).sum()
+    assert abs(F.sum(axis=0)).max() <= (1e-12 if precision == 0 else 1e-6) * scale
+    off, nbr = ctx.neighbor_lists()
+    rows = np.repeat(np.arange(col["n"]), np.diff(off))
+    a = rows.astype(np.int64) * col["n"] + nbr
+    b = nbr.astype(np.int64) * col["n"] + rows
+    assert np.array_equal(np.sort(a), np.sort(b))
+    grid = col["grid"]
+    prod = tm.DeviceArray(ctx, grid["G"] ** 3, np.float64)
+    ctx.cic_deposit(grid, prod, 0)
+    assert abs(prod.get().sum() - col["n"]) <= 1e-9 * col["n"]

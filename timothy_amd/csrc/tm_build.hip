@@ -1,0 +1,424 @@
+// Build stage: per-colony affine map, 29-bit location codes, Morton order,
+// PR-octree leaf octants and the radix tree the neighbour walk descends.
+//
+// Reference: octbuild (src/octree.c:145-207).  The reference inserts cells one
+// by one into a pointer PR-octree (one cell per leaf).  The octree is unique
+// for a set of codes, so it is rebuilt here from the Morton order instead:
+// a cell's leaf octant has side 2^s with s the highest bit in which its codes
+// differ from its closest Morton neighbour, and the binary radix tree over the
+// sorted keys (Karras 2012) has every subtree's leaf octants inside the
+// subtree's key-prefix region, which is what the walk tests.
+#include "tm_internal.h"
+
+namespace {
+
+constexpr int BT = 256;
+
+__device__ __forceinline__ unsigned long long ord_key(double v) {
+  unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unord_key(unsigned long long u) {
+  return __longlong_as_double((long long)((u >> 63) ? (u & 0x7fffffffffffffffull) : ~u));
+}
+
+__device__ __forceinline__ int colony_of(const int64_t *__restrict__ off, int ncol, int64_t i) {
+  int lo = 0, hi = ncol;  // off[lo] <= i < off[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// (uint32)(((x - shift)/scale) * MAXVALUE), src/octree.c:188-190
+__device__ __forceinline__ uint32_t loc_code(double x, double shift, double scale) {
+  return (uint32_t)(((x - shift) / scale) * 536870912.0);
+}
+
+__device__ __forceinline__ uint64_t spread21(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+// top 64 bits of the 87-bit Morton key (level 28 first, z,y,x within a level)
+__device__ __forceinline__ uint64_t morton_prefix64(uint32_t cx, uint32_t cy, uint32_t cz) {
+  uint64_t k63 = spread21(cx >> 8) | (spread21(cy >> 8) << 1) | (spread21(cz >> 8) << 2);
+  return (k63 << 1) | ((cz >> 7) & 1u);
+}
+
+// full-key order of two cells (same colony): true if a < b
+__device__ __forceinline__ bool morton_less(uint4 a, uint4 b) {
+  if (a.w != b.w) return a.w < b.w;
+  int bit = tm_diffbit(a, b);
+  if (bit < 0) return false;
+  uint32_t m = 1u << bit;
+  if ((a.z ^ b.z) & m) return (b.z & m) != 0;
+  if ((a.y ^ b.y) & m) return (b.y & m) != 0;
+  return (b.x & m) != 0;
+}
+
+// common prefix length of (colony id : 16 bits, Morton key : 87 bits)
+__device__ __forceinline__ int key_delta(const uint4 *__restrict__ code, int64_t n, int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  uint4 a = code[i], b = code[j];
+  if (a.w != b.w) return __clz((int)(a.w ^ b.w)) - 16;
+  int bit = tm_diffbit(a, b);
+  if (bit < 0) return TM_CID_BITS + 87 + __clz((int)((uint32_t)i ^ (uint32_t)j));  // duplicate codes (flagged)
+  uint32_t m = 1u << bit;
+  int r = ((a.z ^ b.z) & m) ? 0 : (((a.y ^ b.y) & m) ? 1 : 2);
+  return TM_CID_BITS + 3 * (28 - bit) + r;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v0, unsigned long long v1, int per) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = ((i % per) < 3) ? v0 : v1;
+}
+
+__global__ __launch_bounds__(BT) void k_bbox(const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ z, const int32_t *__restrict__ ctype,
+                                             int64_t n, const int64_t *__restrict__ off, int ncol,
+                                             const TypeDev *__restrict__ types, int ntypes,
+                                             unsigned long long *__restrict__ bbox, unsigned int *__restrict__ flags) {
+  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  bool valid = i < n;
+  int cid = 0;
+  unsigned long long k[6];
+  if (valid) {
+    cid = colony_of(off, ncol, i);
+    int t = ctype[i];
+    if (t < 0 || t >= ntypes) {
+      atomicOr(flags, (unsigned)TM_FLAG_CTYPE);
+      t = 0;
+    }
+    double e = types[t].h + 0.01;  // src/octree.c:167
+    double px = x[i], py = y[i], pz = z[i];
+    k[0] = ord_key(px - e);
+    k[1] = ord_key(py - e);
+    k[2] = ord_key(pz - e);
+    k[3] = ord_key(px + e);
+    k[4] = ord_key(py + e);
+    k[5] = ord_key(pz + e);
+  } else {
+    k[0] = k[1] = k[2] = ~0ull;
+    k[3] = k[4] = k[5] = 0ull;
+  }
+  int cid0 = __shfl(cid, 0, 64);
+  unsigned long long act = __ballot(1);
+  bool uniform = __ballot(!valid || cid == cid0) == act;
+  if (uniform) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        unsigned long long t = __shfl_xor(k[a], o, 64);
+        k[a] = t < k[a] ? t : k[a];
+        unsigned long long u = __shfl_xor(k[a + 3], o, 64);
+        k[a + 3] = u > k[a + 3] ? u : k[a + 3];
+      }
+    }
+    if (__lane_id() == 0) {
+      unsigned long long *b = bbox + 6 * (int64_t)cid0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(b + a, k[a]);
+        atomicMax(b + 3 + a, k[a + 3]);
+      }
+    }
+  } else if (valid) {
+    unsigned long long *b = bbox + 6 * (int64_t)cid;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(b + a, k[a]);
+      atomicMax(b + 3 + a, k[a + 3]);
+    }
+  }
+}
+
+// src/octree.c:175-180
+__global__ void k_affine(const unsigned long long *__restrict__ bbox, int ncol, double4 *__restrict__ aff) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  const unsigned long long *b = bbox + 6 * (int64_t)c;
+  double mn[3], mx[3];
+  for (int a = 0; a < 3; ++a) {
+    mn[a] = unord_key(b[a]);
+    mx[a] = unord_key(b[3 + a]);
+  }
+  double s = mx[0] - mn[0];
+  s = (s > mx[1] - mn[1] ? s : mx[1] - mn[1]);
+  s = (s > mx[2] - mn[2] ? s : mx[2] - mn[2]);
+  aff[c] = make_double4(mn[0], mn[1], mn[2], s);
+}
+
+__global__ __launch_bounds__(BT) void k_keys(const double *__restrict__ x, const double *__restrict__ y,
+                                             const double *__restrict__ z, int64_t n,
+                                             const int64_t *__restrict__ off, int ncol, int cb,
+                                             const double4 *__restrict__ aff, uint64_t *__restrict__ key,
+                                             int32_t *__restrict__ val) {
+  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i >= n) return;
+  int cid = ncol > 1 ? colony_of(off, ncol, i) : 0;
+  double4 a = aff[cid];
+  uint32_t cx = loc_code(x[i], a.x, a.w), cy = loc_code(y[i], a.y, a.w), cz = loc_code(z[i], a.z, a.w);
+  uint64_t k = morton_prefix64(cx, cy, cz);
+  if (cb > 0) k = ((uint64_t)cid << (64 - cb)) | (k >> cb);
+  key[i] = k;
+  val[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm, int64_t n,
+                                               const double *__restrict__ x, const double *__restrict__ y,
+                                               const double *__restrict__ z, const double *__restrict__ size,
+                                               const float *__restrict__ young, const int32_t *__restrict__ ctype,
+                                               const int32_t *__restrict__ phase, const int64_t *__restrict__ off,
+                                               int ncol, const double4 *__restrict__ aff, uint4 *__restrict__ code,
+                                               double4 *__restrict__ pos, float *__restrict__ young_s,
+                                               int32_t *__restrict__ ctype_s, int32_t *__restrict__ phase_s,
+                                               int ntypes) {
+  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i >= n) return;
+  int64_t o = perm[i];
+  if (o < 0 || o >= n) o = 0;  // unreachable for a valid permutation; bounds the gathers
+  int cid = ncol > 1 ? colony_of(off, ncol, o) : 0;
+  double4 a = aff[cid];
+  double px = x[o], py = y[o], pz = z[o];
+  code[i] = make_uint4(loc_code(px, a.x, a.w), loc_code(py, a.y, a.w), loc_code(pz, a.z, a.w), (uint32_t)cid);
+  pos[i] = make_double4(px, py, pz, size[o]);
+  young_s[i] = young ? young[o] : 0.0f;
+  const int32_t t = ctype[o];
+  ctype_s[i] = (t < 0 || t >= ntypes) ? 0 : t;  // out-of-range types were flagged by k_bbox
+  phase_s[i] = phase ? phase[o] : 1;
+}
+
+// Cells whose 64-bit key prefixes tie are put in full 87-bit key order; the
+// first thread of each run sorts it (runs are a few cells at most).
+__global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key, int64_t n, int32_t *perm, uint4 *code,
+                                                double4 *pos, float *young_s, int32_t *ctype_s, int32_t *phase_s,
+                                                unsigned int *flags) {
+  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i >= n - 1) return;
+  if (key[i + 1] != key[i]) return;
+  if (i > 0 && key[i - 1] == key[i]) return;
+  int64_t e = i + 1;
+  while (e < n && key[e] == key[i] && e - i <= 64) ++e;
+  if (e - i > 64) {  // > 64 cells within 2^8 codes on every axis: treated as degenerate
+    atomicOr(flags, (unsigned)TM_FLAG_DEGENERATE);
+    return;
+  }
+  for (int64_t a = i + 1; a < e; ++a) {
+    uint4 c = code[a];
+    int32_t pm = perm[a];
+    double4 ps = pos[a];
+    float yg = young_s[a];
+    int32_t ct = ctype_s[a], ph = phase_s[a];
+    int64_t b = a - 1;
+    while (b >= i && morton_less(c, code[b])) {
+      code[b + 1] = code[b];
+      perm[b + 1] = perm[b];
+      pos[b + 1] = pos[b];
+      young_s[b + 1] = young_s[b];
+      ctype_s[b + 1] = ctype_s[b];
+      phase_s[b + 1] = phase_s[b];
+      --b;
+    }
+    code[b + 1] = c;
+    perm[b + 1] = pm;
+    pos[b + 1] = ps;
+    young_s[b + 1] = yg;
+    ctype_s[b + 1] = ct;
+    phase_s[b + 1] = ph;
+  }
+}
+
+// leaf octant exponent: min over the same-colony Morton neighbours of the
+// highest differing code bit (the level at which the PR-octree separates the
+// cell); 29 (the root) for a colony of one.  Flags duplicate codes and splits
+// below node level 3, where the reference shifts by negative counts
+// (src/octree.c:105-107).
+__global__ __launch_bounds__(BT) void k_leaf(const uint4 *__restrict__ code, int64_t n, int32_t *__restrict__ span,
+                                             unsigned int *__restrict__ flags) {
+  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i >= n) return;
+  uint4 me = code[i];
+  int best = 99;
+  bool bad = false;
+  if (i > 0) {
+    uint4 o = code[i - 1];
+    if (o.w == me.w) {
+      int b = tm_diffbit(me, o);
+      if (b < 0) bad = true;
+      best = min(best, b);
+    }
+  }
+  if (i + 1 < n) {
+    uint4 o = code[i + 1];
+    if (o.w == me.w) {
+      int b = tm_diffbit(me, o);
+      if (b < 0) bad = true;
+      best = min(best, b);
+    }
+  }
+  if (best == 99) best = 29;
+  if (best <= 1) bad = true;
+  if (bad) atomicOr(flags, (unsigned)TM_FLAG_DEGENERATE);
+  span[i] = best < 0 ? 0 : best;
+}
+
+// Karras (2012) radix tree over (colony, Morton) keys: internal node i's key
+// range, common-prefix length, region and the child the descending walk
+// visits first (the right one).
+__global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, int64_t n, int2 *__restrict__ range,
+                                               int32_t *__restrict__ delta, int32_t *__restrict__ split,
+                                               NodeRec *__restrict__ nodes) {
+  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (key_delta(code, n, i, i + 1) - key_delta(code, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = key_delta(code, n, i, i - d);
+  int64_t lmax = 2;
+  while (key_delta(code, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (key_delta(code, n, i, i + (l + t) * d) > dmin) l += t;
+  int64_t j = i + l * d;
+  int dnode = key_delta(code, n, i, j);
+  int64_t s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (key_delta(code, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+  int64_t first = min(i, j), last = max(i, j);
+  range[i] = make_int2((int)first, (int)last);
+  delta[i] = dnode;
+  split[i] = (int32_t)gamma;
+  NodeRec r;
+  uint4 c0 = code[first];
+  if (dnode >= TM_CID_BITS) {
+    int dm = dnode - TM_CID_BITS;
+    int k = dm / 3, rr = dm % 3;
+    int nb[3] = {k, k + (rr >= 2 ? 1 : 0), k + (rr >= 1 ? 1 : 0)};  // x, y, z fixed bits
+    uint32_t cc[3] = {c0.x, c0.y, c0.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      uint32_t side = 1u << (29 - nb[a]);
+      r.lo[a] = cc[a] & ~(side - 1u);
+      r.hi[a] = r.lo[a] + side;
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      r.lo[a] = 0;
+      r.hi[a] = 1u << 29;
+    }
+  }
+  r.child = (last == gamma + 1) ? (uint32_t)((n - 1) + gamma + 1) : (uint32_t)(gamma + 1);
+  r.esc = TM_END;
+  nodes[i] = r;
+}
+
+// escape links of the descending walk for every node (internal and leaf),
+// and the leaf octants: after the subtree covering [a, b], the walk goes to
+// the left child of the split between a-1 and a, which is internal node a-1
+// when that node ends at a-1, else leaf a-1; nothing after a colony's first cell.
+__global__ __launch_bounds__(BT) void k_escape(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
+                                               const int2 *__restrict__ range, int64_t n,
+                                               NodeRec *__restrict__ nodes) {
+  int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (k >= 2 * n - 1) return;
+  int64_t a;
+  if (k < n - 1) {
+    a = range[k].x;
+  } else {
+    a = k - (n - 1);
+    uint4 c = code[a];
+    int s = span[a];
+    uint32_t side = (s >= 29) ? (1u << 29) : (1u << s);
+    NodeRec r;
+    r.lo[0] = c.x & ~(side - 1u);
+    r.lo[1] = c.y & ~(side - 1u);
+    r.lo[2] = c.z & ~(side - 1u);
+    r.hi[0] = r.lo[0] + side;
+    r.hi[1] = r.lo[1] + side;
+    r.hi[2] = r.lo[2] + side;
+    r.child = (uint32_t)a;
+    r.esc = TM_END;
+    nodes[k] = r;
+  }
+  uint32_t esc = TM_END;
+  if (a > 0 && code[a - 1].w == code[a].w) {
+    int64_t j = a - 1;
+    if (j <= n - 2 && range[j].y == j) esc = (uint32_t)j;
+    else esc = (uint32_t)((n - 1) + j);
+  }
+  nodes[k].esc = esc;
+}
+
+__global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 *__restrict__ range, int64_t n,
+                          int32_t *__restrict__ root, unsigned int *__restrict__ flags) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncol) return;
+  int64_t s = off[c], e = off[c + 1] - 1;
+  int32_t r = -1;
+  if (e == s) {
+    r = (int32_t)((n - 1) + s);
+  } else if (e > s) {
+    if (s <= n - 2 && range[s].x == s && range[s].y == e) r = (int32_t)s;
+    else if (e <= n - 2 && range[e].x == s && range[e].y == e) r = (int32_t)e;
+    else atomicOr(flags, (unsigned)TM_FLAG_ROOT);
+  }
+  root[c] = r;
+}
+
+__global__ void k_zero_u32(unsigned int *p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace
+
+tm_status tm_run_build(tm_ctx *c) {
+  const int64_t n = c->n;
+  if (c->stage < 1) return tm_fail(c, TM_ERR_STATE, "tm_build before tm_upload");
+  if (n == 0) {
+    c->stage = 2;
+    return TM_OK;
+  }
+  hipStream_t st = c->stream;
+  k_zero_u32<<<1, 64, 0, st>>>(c->d_flags, 4);
+  // bounding boxes and affine maps
+  int64_t nb6 = 6 * (int64_t)c->ncol;
+  k_fill_u64<<<tm_blocks(nb6, 256), 256, 0, st>>>(c->d_bbox, nb6, ~0ull, 0ull, 6);
+  k_bbox<<<tm_blocks(n, BT), BT, 0, st>>>(c->x, c->y, c->z, c->ctype, n, c->d_col_off, c->ncol, c->d_types,
+                                           c->ntypes, c->d_bbox, c->d_flags);
+  k_affine<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_bbox, c->ncol, c->d_aff);
+  TM_HIP(c, hipGetLastError());
+  // Morton keys and the sort
+  int cb = 0;
+  while ((1ll << cb) < c->ncol) ++cb;
+  k_keys<<<tm_blocks(n, BT), BT, 0, st>>>(c->x, c->y, c->z, n, c->d_col_off, c->ncol, cb, c->d_aff, c->d_key,
+                                           c->d_perm);
+  TM_HIP(c, hipGetLastError());
+  TM_TRY(tm_sort_pairs(c, c->d_key, c->d_perm, n));
+  k_gather<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_perm, n, c->x, c->y, c->z, c->size, c->young, c->ctype, c->phase,
+                                             c->d_col_off, c->ncol, c->d_aff, c->d_code, c->d_pos, c->d_young_s,
+                                             c->d_ctype_s, c->d_phase_s, c->ntypes);
+  k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_young_s,
+                                              c->d_ctype_s, c->d_phase_s, c->d_flags);
+  k_leaf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, n, c->d_span, c->d_flags);
+  TM_HIP(c, hipGetLastError());
+  // radix tree
+  if (n > 1)
+    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, n, c->d_range, c->d_delta, c->d_split, c->d_nodes);
+  k_escape<<<tm_blocks(2 * n - 1, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, n, c->d_nodes);
+  k_colroot<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_col_off, c->ncol, c->d_range, n, c->d_colroot, c->d_flags);
+  TM_HIP(c, hipGetLastError());
+  c->stage = 2;
+  return TM_OK;
+}

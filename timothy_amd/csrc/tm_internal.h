@@ -1,0 +1,191 @@
+// Internal declarations of the MI355X cell-mechanics library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/timothy_hip.h"
+
+#define TM_MAX_TYPES 16
+#define TM_END 0xFFFFFFFFu
+#define TM_CID_BITS 16
+#define TM_FLAG_DEGENERATE 1
+#define TM_FLAG_ROOT 2
+#define TM_FLAG_CTYPE 4
+#define TM_FLAG_WALK 8
+
+// Per cell-type constants, derived on the host with the reference's own
+// expressions (src/inline.h:45-46,71-72) so the device reads identical values.
+struct TypeDev {
+  double size, h, h2, h3, h4;
+  double sc;       // h3 in 3D, h2 in 2D: density scale (src/potential.c:58)
+  double cW;       // kernel normalisation 8/(pi h3) (3D) or 40/(7 pi h2) (2D)
+  double cF;       // gradient normalisation 48/(pi h4) (3D) or 320/(7 pi h3) (2D)
+  double cut2;     // largest d2 with RN(sqrt(d2)) <= h: exact form of dist <= h
+  float hinv_f, cW_f, cF_f, sc_over_size_f;
+};
+
+// 32-byte tree node: the region [lo, hi) in location-code units per axis,
+// the child visited first by the descending walk (internal nodes) and the
+// escape to the next node in that walk (TM_END after a colony's last node).
+struct NodeRec {
+  uint32_t lo[3];
+  uint32_t child;
+  uint32_t hi[3];
+  uint32_t esc;
+};
+
+struct tm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // configuration
+  int ntypes = 0;
+  int dim = 3;
+  tm_celltype types[TM_MAX_TYPES];
+  TypeDev *d_types = nullptr;
+
+  // tunables
+  int64_t nbr_cap = 48;
+  int sph_block = 256;
+  int nbr_block = 256;
+  int sph_lanes = 1;
+  int mixed_recentre = 0;
+
+  // cells (reference order)
+  int64_t n = 0, cap_n = 0;
+  int ncol = 0;
+  std::vector<int64_t> col_off;
+  bool external = false;
+  const double *x = nullptr, *y = nullptr, *z = nullptr, *size = nullptr;
+  const float *young = nullptr;
+  const int32_t *ctype = nullptr, *phase = nullptr;
+  double *own_x = nullptr, *own_y = nullptr, *own_z = nullptr, *own_size = nullptr;
+  float *own_young = nullptr;
+  int32_t *own_ctype = nullptr, *own_phase = nullptr;
+  int64_t own_cap = 0;
+  int64_t *d_col_off = nullptr;
+  int64_t cap_col = 0;
+
+  // per colony
+  double4 *d_aff = nullptr;              // shift.xyz, scale
+  unsigned long long *d_bbox = nullptr;  // 6 ordered keys per colony
+  int32_t *d_colroot = nullptr;
+
+  // sort scratch
+  uint64_t *d_key = nullptr, *d_key_alt = nullptr;
+  int32_t *d_val_alt = nullptr;
+  uint32_t *d_hist = nullptr, *d_scan = nullptr;
+  int64_t cap_hist = 0, cap_scan = 0;
+
+  // Morton order
+  int32_t *d_perm = nullptr;
+  uint4 *d_code = nullptr;  // x, y, z codes, colony id
+  double4 *d_pos = nullptr; // x, y, z, size
+  float4 *d_posf = nullptr; // block-recentred FP32 positions, size
+  double4 *d_origin = nullptr; // per 64-cell block origin (xyz) and extent flag (w)
+  float *d_young_s = nullptr;
+  int32_t *d_ctype_s = nullptr, *d_phase_s = nullptr;
+  int32_t *d_span = nullptr;
+
+  // radix tree
+  NodeRec *d_nodes = nullptr;
+  int2 *d_range = nullptr;
+  int32_t *d_delta = nullptr;
+  int32_t *d_split = nullptr;   // split position of each internal node
+  int32_t *d_parent = nullptr;  // parent of every node (2n-1), -1 at roots
+  int64_t cap_tree_aux = 0;
+
+  // neighbours (slot-major: nbr[k * n + p])
+  uint32_t *d_nbr = nullptr;
+  int64_t nbr_alloc = 0;  // slots per cell of the current layout
+  int64_t nbr_elems = 0;  // elements allocated (>= n * nbr_alloc)
+  uint32_t *d_ncount = nullptr;
+  int32_t *d_qstar = nullptr;
+
+  // outputs, Morton order
+  double *d_dens = nullptr, *d_v = nullptr, *d_mind = nullptr;
+  double *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr;
+  // unpermuted outputs
+  double *d_out = nullptr;
+
+  // flags: [0] error bits, [1] max neighbour count, [2] pairs low, [3] pairs high
+  unsigned int *d_flags = nullptr;
+  unsigned int *h_flags = nullptr;  // pinned
+
+  int stage = 0;  // 0 empty, 1 cells, 2 built, 3 neighbours, 4 sph
+  tm_precision last_prec = TM_FP64;
+
+  // last deposit, for timing
+  tm_grid last_grid{};
+  int last_weight = 0;
+  double *last_prod = nullptr;
+
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
+
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  tm_precision graph_prec = TM_FP64;
+};
+
+// error helpers
+tm_status tm_fail(tm_ctx *c, tm_status s, const std::string &msg);
+#define TM_HIP(c, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return tm_fail((c), TM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+#define TM_TRY(expr)                 \
+  do {                               \
+    tm_status _s = (expr);           \
+    if (_s != TM_OK) return _s;      \
+  } while (0)
+
+template <typename T>
+tm_status tm_ensure(tm_ctx *c, T **p, int64_t &cap, int64_t need, const char *what) {
+  if (need <= cap && *p) return TM_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  cap = 0;
+  if (need <= 0) return TM_OK;
+  if (hipMalloc((void **)p, sizeof(T) * (size_t)need) != hipSuccess) {
+    *p = nullptr;
+    return tm_fail(c, TM_ERR_NOMEM, std::string("hipMalloc failed: ") + what);
+  }
+  cap = need;
+  return TM_OK;
+}
+
+// stage entry points implemented per file
+tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n);
+tm_status tm_scan_u32(tm_ctx *c, const uint32_t *in, uint32_t *out, int64_t n);
+tm_status tm_run_build(tm_ctx *c);
+tm_status tm_run_neighbors(tm_ctx *c, bool check);
+tm_status tm_run_sph(tm_ctx *c, tm_precision prec);
+tm_status tm_run_unpermute(tm_ctx *c);
+tm_status tm_run_deposit(tm_ctx *c, const tm_grid *g, int weight_mode, double *prod);
+tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count);
+
+// small device helpers ------------------------------------------------------
+__device__ __forceinline__ unsigned long long tm_lanemask_lt() {
+  unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ int tm_msb32(uint32_t v) { return 31 - __clz((int)v); }
+
+// highest differing bit of two code triples (-1 if equal)
+__device__ __forceinline__ int tm_diffbit(uint4 a, uint4 b) {
+  uint32_t o = (a.x ^ b.x) | (a.y ^ b.y) | (a.z ^ b.z);
+  return o ? tm_msb32(o) : -1;
+}
+
+inline unsigned tm_blocks(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }

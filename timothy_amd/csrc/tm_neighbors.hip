@@ -1,0 +1,145 @@
+// Neighbour stage: one lane per cell walks its colony's radix tree with the
+// reference's query box and strict overlap test, visiting leaf octants in
+// descending Morton key -- the order the reference's LIFO DFS pops them
+// (children pushed 0..7, src/potential.c:111-131).
+//
+// Per cell p this yields:
+//   N(p)    candidates q != p with sqrt((dx*dx+dy*dy)+dz*dz) <= h_p, stored in
+//           visit order in slot-major lists nbr[k*n + p] (coalesced for the
+//           one-lane-per-cell pair kernels), count[p]
+//   mindist min distance over all candidates (src/potential.c:49), computed as
+//           sqrt of the minimum squared distance (sqrt is monotone and
+//           correctly rounded, so this is the minimum of the rounded roots)
+//   qstar   the last candidate visited = minimum key, the one whose potential
+//           the reference leaves in v (src/potential.c:124 assigns, not adds)
+// The cutoff test d <= h is evaluated exactly as d2 <= cut2, cut2 being the
+// largest double whose correctly rounded root is <= h.
+#include "tm_internal.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t box_code(double v, double shift, double scale) {
+  return (uint32_t)((v - shift) / scale * 536870912.0);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
+                                                  const int32_t *__restrict__ ctype_s,
+                                                  const NodeRec *__restrict__ nodes,
+                                                  const int32_t *__restrict__ colroot,
+                                                  const double4 *__restrict__ aff,
+                                                  const TypeDev *__restrict__ types, int64_t n, int64_t cap,
+                                                  uint32_t *__restrict__ nbr, uint32_t *__restrict__ count,
+                                                  double *__restrict__ mindist, int32_t *__restrict__ qstar,
+                                                  unsigned int *__restrict__ flags) {
+  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (p >= n) return;
+  const double4 P = pos[p];
+  const uint32_t cid = code[p].w;
+  const double4 A = aff[cid];
+  const TypeDev &T = types[ctype_s[p]];
+  const double h = T.h;
+  const double cut2 = T.cut2;
+  // query box, src/octree.c:246-266: ((x - h) - shift) / scale * 2^29
+  const uint32_t mn0 = box_code(P.x - h, A.x, A.w), mx0 = box_code(P.x + h, A.x, A.w);
+  const uint32_t mn1 = box_code(P.y - h, A.y, A.w), mx1 = box_code(P.y + h, A.y, A.w);
+  const uint32_t mn2 = box_code(P.z - h, A.z, A.w), mx2 = box_code(P.z + h, A.z, A.w);
+  const uint32_t leaf0 = (uint32_t)(n - 1);
+  const uint32_t nnodes = (uint32_t)(2 * n - 1);
+  uint32_t node = (uint32_t)colroot[cid];
+  uint32_t cnt = 0;
+  int32_t qs = -1;
+  double md2 = 1.0 / 0.0;
+  // a well-formed walk visits each node at most once; the budget only stops a
+  // malformed tree from looping
+  int64_t budget = 2 * n + 8;
+  while (node != TM_END) {
+    if (node >= nnodes || --budget < 0) {
+      atomicOr(flags, (unsigned)TM_FLAG_WALK);
+      break;
+    }
+    const uint4 a = *reinterpret_cast<const uint4 *>(&nodes[node].lo[0]);
+    const uint4 b = *reinterpret_cast<const uint4 *>(&nodes[node].hi[0]);
+    const bool hit = (mx0 > a.x) & (mn0 < b.x) & (mx1 > a.y) & (mn1 < b.y) & (mx2 > a.z) & (mn2 < b.z);
+    if (hit && node < leaf0) {
+      node = a.w;
+      continue;
+    }
+    if (hit) {
+      const uint32_t q = node - leaf0;
+      if (q != (uint32_t)p) {
+        const double4 Q = pos[q];
+        const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+        const double d2 = (dx * dx + dy * dy) + dz * dz;
+        md2 = d2 < md2 ? d2 : md2;
+        qs = (int32_t)q;
+        if (d2 <= cut2) {
+          if (cnt < cap) nbr[(int64_t)cnt * n + p] = q;
+          ++cnt;
+        }
+      }
+    }
+    node = b.w;
+  }
+  count[p] = cnt;
+  mindist[p] = qs >= 0 ? sqrt(md2) : 1.7976931348623157e308;
+  qstar[p] = qs;
+  if (cnt > cap) atomicMax(flags + 1, cnt);
+}
+
+}  // namespace
+
+// grows the neighbour lists and repeats the walk when a cell overflowed them
+tm_status tm_run_neighbors(tm_ctx *c, bool check) {
+  if (c->stage < 2) return tm_fail(c, TM_ERR_STATE, "tm_neighbors before tm_build");
+  const int64_t n = c->n;
+  if (n == 0) {
+    c->stage = 3;
+    return TM_OK;
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const int64_t need = n * c->nbr_cap;
+    if (c->nbr_elems < need || !c->d_nbr) {
+      if (c->d_nbr) (void)hipFree(c->d_nbr);
+      c->d_nbr = nullptr;
+      c->nbr_elems = 0;
+      if (hipMalloc((void **)&c->d_nbr, sizeof(uint32_t) * (size_t)need) != hipSuccess) {
+        c->d_nbr = nullptr;
+        return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed: neighbour lists");
+      }
+      c->nbr_elems = need;
+    }
+    c->nbr_alloc = c->nbr_cap;
+    const int bs = c->nbr_block;
+    if (bs == 64)
+      k_neighbors<64><<<tm_blocks(n, 64), 64, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
+                                                               c->d_colroot, c->d_aff, c->d_types, n, c->nbr_alloc,
+                                                               c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar,
+                                                               c->d_flags);
+    else if (bs == 128)
+      k_neighbors<128><<<tm_blocks(n, 128), 128, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
+                                                                  c->d_colroot, c->d_aff, c->d_types, n,
+                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind,
+                                                                  c->d_qstar, c->d_flags);
+    else
+      k_neighbors<256><<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
+                                                                  c->d_colroot, c->d_aff, c->d_types, n,
+                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind,
+                                                                  c->d_qstar, c->d_flags);
+    TM_HIP(c, hipGetLastError());
+    if (!check) break;
+    TM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    TM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->h_flags[0] & TM_FLAG_DEGENERATE)
+      return tm_fail(c, TM_ERR_DEGENERATE,
+                     "two cells share location codes down to bit 1 (duplicate or near-duplicate positions)");
+    if (c->h_flags[0] & TM_FLAG_ROOT) return tm_fail(c, TM_ERR_STATE, "colony root not found in the radix tree");
+    if (c->h_flags[0] & TM_FLAG_WALK) return tm_fail(c, TM_ERR_STATE, "malformed tree: walk left its node range");
+    if (c->h_flags[0] & TM_FLAG_CTYPE) return tm_fail(c, TM_ERR_INVALID, "ctype out of range");
+    const int64_t mx = c->h_flags[1];
+    if (mx <= c->nbr_alloc) break;
+    c->nbr_cap = (mx + 7) & ~7ll;  // grow to the largest count seen, rounded up
+  }
+  c->stage = 3;
+  return TM_OK;
+}

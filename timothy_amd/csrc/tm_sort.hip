@@ -1,0 +1,227 @@
+// Device-wide exclusive scan and stable LSD radix sort of (u64 key, i32 value)
+// pairs, 8-bit digits, for the Morton ordering of cells.
+//
+// Each pass: histogram per 4096-key tile -> exclusive scan of the digit-major
+// histogram -> stable scatter.  Stable ranks inside a tile come from 64-lane
+// ballots: the lanes holding the same digit are found with eight ballots,
+// the leader of each group advances a per-wave digit counter in LDS.
+#include "tm_internal.h"
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 8;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+// lanes of this wave that are valid and hold the same 8-bit digit
+__device__ __forceinline__ unsigned long long tm_match8(uint32_t d, bool valid) {
+  unsigned long long m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    bool bit = (d >> b) & 1u;
+    unsigned long long bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
+                                                        uint32_t *__restrict__ hist, int nblocks) {
+  __shared__ uint32_t h[RS_WAVES][256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int j = tid; j < RS_WAVES * 256; j += RS_THREADS) (&h[0][0])[j] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    int64_t i = base + it * 64 + lane;
+    bool valid = i < n;
+    uint32_t d = valid ? (uint32_t)((keys[i] >> shift) & 255u) : 0u;
+    unsigned long long peers = tm_match8(d, valid);
+    if (valid && (peers & tm_lanemask_lt()) == 0) h[w][d] += (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < RS_WAVES; ++k) s += h[k][tid];
+  hist[(int64_t)tid * nblocks + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ keys,
+                                                           const int32_t *__restrict__ vals,
+                                                           uint64_t *__restrict__ okeys, int32_t *__restrict__ ovals,
+                                                           int64_t n, int shift, const uint32_t *__restrict__ offs,
+                                                           int nblocks) {
+  __shared__ uint32_t cnt[RS_WAVES][256];
+  __shared__ uint32_t pre[RS_WAVES][256];
+  __shared__ uint32_t gbase[256];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int j = tid; j < RS_WAVES * 256; j += RS_THREADS) (&cnt[0][0])[j] = 0;
+  gbase[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
+  uint64_t k[RS_ITEMS];
+  int32_t v[RS_ITEMS];
+  uint32_t rank[RS_ITEMS];
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    int64_t i = base + it * 64 + lane;
+    bool valid = i < n;
+    k[it] = valid ? keys[i] : 0ull;
+    v[it] = valid ? vals[i] : 0;
+    uint32_t d = (uint32_t)((k[it] >> shift) & 255u);
+    unsigned long long peers = tm_match8(d, valid);
+    uint32_t before = (uint32_t)__popcll(peers & tm_lanemask_lt());
+    uint32_t c0 = valid ? cnt[w][d] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    if (valid && before == 0) cnt[w][d] = c0 + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    rank[it] = c0 + before;
+  }
+  __syncthreads();
+  {
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q) {
+      pre[q][tid] = run;
+      run += cnt[q][tid];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    int64_t i = base + it * 64 + lane;
+    if (i < n) {
+      uint32_t d = (uint32_t)((k[it] >> shift) & 255u);
+      int64_t pos = (int64_t)gbase[d] + pre[w][d] + rank[it];
+      if (pos < n) {  // always true for a consistent histogram; keeps a bad scan from writing out of bounds
+        okeys[pos] = k[it];
+        ovals[pos] = v[it];
+      }
+    }
+  }
+}
+
+// ---- scan ----------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
+// exclusive block scan of one value per thread; returns the block total in *tot
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *tot) {
+  __shared__ uint32_t ws[SC_THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(x);
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  uint32_t wbase = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < SC_THREADS / 64; ++k) {
+    uint32_t s = ws[k];
+    if (k < w) wbase += s;
+    all += s;
+  }
+  __syncthreads();
+  *tot = all;
+  return wbase + inc - x;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(const uint32_t *__restrict__ in, int64_t n,
+                                                            uint32_t *__restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SC_ITEMS; ++k)
+    if (base + k < n) s += in[base + k];
+  uint32_t tot;
+  (void)block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_apply(const uint32_t *__restrict__ in, int64_t n,
+                                                           const uint32_t *__restrict__ sums_scanned,
+                                                           uint32_t *__restrict__ out) {
+  const int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+  uint32_t v[SC_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SC_ITEMS; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan(s, &tot) + (sums_scanned ? sums_scanned[blockIdx.x] : 0u);
+#pragma unroll
+  for (int k = 0; k < SC_ITEMS; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+}  // namespace
+
+// exclusive scan; `out` may alias `in`.  Uses c->d_scan as scratch.
+tm_status tm_scan_u32(tm_ctx *c, const uint32_t *in, uint32_t *out, int64_t n) {
+  if (n <= 0) return TM_OK;
+  int64_t nb = (n + SC_TILE - 1) / SC_TILE;
+  if (nb == 1) {
+    k_scan_apply<<<1, SC_THREADS, 0, c->stream>>>(in, n, nullptr, out);
+    TM_HIP(c, hipGetLastError());
+    return TM_OK;
+  }
+  // scratch layout: level sums for up to three levels
+  int64_t nb2 = (nb + SC_TILE - 1) / SC_TILE;
+  int64_t need = nb + nb2 + 1;
+  TM_TRY(tm_ensure(c, &c->d_scan, c->cap_scan, need, "scan scratch"));
+  uint32_t *sums = c->d_scan;
+  uint32_t *sums2 = c->d_scan + nb;
+  k_scan_reduce<<<(unsigned)nb, SC_THREADS, 0, c->stream>>>(in, n, sums);
+  TM_HIP(c, hipGetLastError());
+  if (nb2 == 1) {
+    k_scan_apply<<<1, SC_THREADS, 0, c->stream>>>(sums, nb, nullptr, sums);
+  } else {
+    if (nb2 > SC_TILE) return tm_fail(c, TM_ERR_INVALID, "scan too large");
+    k_scan_reduce<<<(unsigned)nb2, SC_THREADS, 0, c->stream>>>(sums, nb, sums2);
+    k_scan_apply<<<1, SC_THREADS, 0, c->stream>>>(sums2, nb2, nullptr, sums2);
+    k_scan_apply<<<(unsigned)nb2, SC_THREADS, 0, c->stream>>>(sums, nb, sums2, sums);
+  }
+  TM_HIP(c, hipGetLastError());
+  k_scan_apply<<<(unsigned)nb, SC_THREADS, 0, c->stream>>>(in, n, sums, out);
+  TM_HIP(c, hipGetLastError());
+  return TM_OK;
+}
+
+// Stable sort of (keys, vals) by all 64 key bits; the result ends in the
+// input arrays (eight passes ping-pong through c->d_key_alt / d_val_alt).
+tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n) {
+  if (n <= 1) return TM_OK;
+  int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+  TM_TRY(tm_ensure(c, &c->d_hist, c->cap_hist, (int64_t)256 * nblocks, "radix histogram"));
+  uint64_t *k0 = keys, *k1 = c->d_key_alt;
+  int32_t *v0 = vals, *v1 = c->d_val_alt;
+  for (int shift = 0; shift < 64; shift += 8) {
+    k_rs_hist<<<nblocks, RS_THREADS, 0, c->stream>>>(k0, n, shift, c->d_hist, nblocks);
+    TM_HIP(c, hipGetLastError());
+    TM_TRY(tm_scan_u32(c, c->d_hist, c->d_hist, (int64_t)256 * nblocks));
+    k_rs_scatter<<<nblocks, RS_THREADS, 0, c->stream>>>(k0, v0, k1, v1, n, shift, c->d_hist, nblocks);
+    TM_HIP(c, hipGetLastError());
+    uint64_t *tk = k0;
+    k0 = k1;
+    k1 = tk;
+    int32_t *tv = v0;
+    v0 = v1;
+    v1 = tv;
+  }
+  return TM_OK;
+}
