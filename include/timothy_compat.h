@@ -1,0 +1,225 @@
+/*
+ * timothy_compat.h -- the reference's cell-mechanics C API, served by the
+ * MI355X path (libtimothy_compat.so over libtimothy_hip.so).
+ *
+ * The structures below must have the reference's memory layout (they cross
+ * the boundary by pointer and by value), so their members are those of
+ * mcytu/timothy src/global.h; the functions have the signatures of
+ * src/octree.h, src/potential.h, src/patches.h and src/fieldgradient.h.  A
+ * program built against the reference headers links this library in place of
+ * octree.c, potential.c, patches.c and fieldgradient.c.
+ *
+ * MPI types: when <mpi.h> is included before this header the structures use
+ * MPI_Comm / MPI_Request; otherwise they use int, which is MPICH's layout.
+ * Only one rank is served (systeminfo.size == 1): the single-GPU build has no
+ * halo cells, so the remote-contribution calls are no-ops when
+ * cellcommdata.numimp <= 0 and an error otherwise.
+ *
+ * Drop-in contract (same as the reference's octree): octbuild snapshots the
+ * cells' positions, sizes, types, young's moduli and phases; the following
+ * computepotential / computegradient / patches_cells2env* calls of the step
+ * use that snapshot.  computepotential evaluates density, v, mindist and the
+ * forces in one pass; computegradient returns those forces.
+ */
+#ifndef TIMOTHY_COMPAT_H
+#define TIMOTHY_COMPAT_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(MPI_VERSION)
+typedef MPI_Comm tm_mpi_comm_t;
+typedef MPI_Request tm_mpi_request_t;
+#else
+typedef int tm_mpi_comm_t;
+typedef int tm_mpi_request_t;
+#endif
+
+#ifndef ZOLTAN_ID_TYPE
+#define ZOLTAN_ID_TYPE int
+#endif
+
+/* src/global.h:90-98 */
+typedef struct cellcount_t {
+  uint64_t n, g0phase, g1phase, sphase, g2phase, mphase, necroticphase;
+} cellcount_t;
+
+/* src/global.h:100-120: 112 bytes on x86-64 */
+typedef struct celldata_t {
+  int lifetime, phase, age, death;
+  float phasetime, g1, s, g2, m, young;
+  ZOLTAN_ID_TYPE gid;
+  double x, y, z, size, v, density, mindist;
+  int ctype;
+} celldata_t;
+
+/* src/global.h:122-151 */
+typedef struct cellenvdata_t { double value, gx, gy, gz; } cellenvdata_t;
+typedef struct double3dv_t { double x, y, z; } double3dv_t;
+typedef struct float3dv_t { float x, y, z; } float3dv_t;
+typedef struct int643dv_t { int64_t x, y, z; } int643dv_t;
+typedef struct uint3dv_t { unsigned int x, y, z; } uint3dv_t;
+
+/* src/global.h:153-164: 112 bytes */
+typedef struct octnode_t {
+  unsigned int xcode, ycode, zcode, xlimit, ylimit, zlimit, level;
+  int64_t father;
+  int64_t child[8];
+  int data;
+} octnode_t;
+
+/* src/global.h:166-179 */
+typedef struct cellsinfo_t {
+  cellcount_t localcount, globalcount;
+  cellcount_t *localtypecount, *globaltypecount;
+  uint64_t *cellsperproc;
+  celldata_t *cells;
+  double3dv_t *forces;
+  octnode_t *octree;
+  int64_t octsize, octmaxsize;
+  int dimension;
+  double3dv_t *velocity;
+} cellsinfo_t;
+
+/* src/global.h:182-190 */
+typedef struct grid_t {
+  int643dv_t globalsize;
+  double3dv_t lowercorner, uppercorner;
+  int643dv_t localsize;
+  float resolution;
+  int643dv_t *loweridx, *upperidx;
+  double3dv_t *lowleftnear, *uprightfar;
+} grid_t;
+
+/* src/global.h:212-215, 247-276 */
+typedef struct explist_t { int64_t cell; int proc; } explist_t;
+typedef struct cellindata_t { double x, y, z, size, young; int ctype; } cellindata_t;
+typedef struct celloutdata_t { double v, density; } celloutdata_t;
+typedef struct cellcommdata_t {
+  explist_t *explist;
+  int explistmaxsize;
+  tm_mpi_request_t *reqsend, *reqrecv;
+  int64_t *sendoffset, *recvoffset;
+  int *recvcount, *sendcount;
+  cellindata_t *sendcellindata, *recvcellindata;
+  celloutdata_t *sendcelloutdata, *recvcelloutdata;
+  int numexp, numimp;
+} cellcommdata_t;
+
+/* src/global.h:278-298 */
+typedef struct settings_t {
+  int64_t maxcells, maxlocalcells;
+  int numberofsteps;
+  float secondsperstep;
+  int numberofcelltypes, numberoffields, dimension, restart;
+  char rstfilename[128], outdir[128];
+  int visoutstep, statoutstep, rstoutstep;
+  float maxspeed, gfdt, gfh;
+  int simulationstart;
+  unsigned int rseed;
+  int step;
+} settings_t;
+
+/* src/global.h:300-312 */
+typedef struct systeminfo_t {
+  int rank, size, nthreads;
+  int dim[3];
+  int noderank, nodesize, memperproc;
+  tm_mpi_comm_t MPI_CART_COMM;
+  int **coords;
+  int endian;
+} systeminfo_t;
+
+/* src/global.h:315-335 */
+typedef struct celltype_t {
+  char name[128];
+  float g1, s, g2, m, v, rd;
+  double size, h, h2, h3, h4;
+  char inputfile[128];
+  float criticaldensity;
+  float *production, *consumption, *criticallevel1, *criticallevel2;
+  float randommove;
+} celltype_t;
+
+/* src/global.h:337-352 */
+typedef struct patches_t {
+  double **data;
+  int *intersect, *receiver, *sender;
+  double **recvdata;
+  tm_mpi_request_t *reqsend, *reqrecv;
+  int643dv_t *lowercorner, *uppercorner, *lowercornerR, *uppercornerR, *size;
+  double **commbuff, **buff;
+} patches_t;
+
+/* src/global.h:354-364 */
+typedef struct environment_t {
+  char name[128];
+  double diffusioncoefficient, boundarycondition, initialconditionmean, initialconditionvariance, lambdadelay;
+  double *data, *gradient, *production;
+} environment_t;
+
+/* src/global.h:408-412 */
+typedef struct octheap_t { int size, count; int *data; } octheap_t;
+
+/* ---- src/octree.h ---- */
+extern double3dv_t affShift;
+extern double affScale;
+void octbuild(systeminfo_t systeminfo, cellsinfo_t *cellsinfo, celltype_t *celltype);
+void octfree(cellsinfo_t *cellsinfo);
+void octheapinit(systeminfo_t systeminfo, octheap_t *ttheap);
+void octcomputebox(int64_t c, uint3dv_t *minLocCode, uint3dv_t *maxLocCode, cellsinfo_t cellsinfo,
+                   celltype_t *celltype);
+int octlocateregion(uint3dv_t minLocCode, uint3dv_t maxLocCode, cellsinfo_t cellsinfo);
+void octheappush(systeminfo_t systeminfo, octheap_t *ttheap, int idx);
+int octheappop(octheap_t *ttheap);
+void octheapfree(octheap_t *ttheap);
+void octcomputeboxr(int64_t c, uint3dv_t *minLocCode, uint3dv_t *maxLocCode, cellcommdata_t cellcommdata,
+                    celltype_t *celltype);
+
+/* ---- src/potential.h ---- */
+void computepotential(systeminfo_t systeminfo, cellsinfo_t *cellsinfo, celltype_t *celltype,
+                      cellcommdata_t cellcommdata);
+void computeremotepotential(systeminfo_t systeminfo, cellsinfo_t *cellsinfo, celltype_t *celltype,
+                            cellcommdata_t cellcommdata);
+void computegradient(systeminfo_t systeminfo, cellsinfo_t *cellsinfo, celltype_t *celltype,
+                     cellcommdata_t cellcommdata);
+void computeremotegradient(systeminfo_t systeminfo, cellsinfo_t *cellsinfo, celltype_t *celltype,
+                           cellcommdata_t cellcommdata);
+
+/* ---- src/patches.h ---- */
+void patches_alloc(systeminfo_t systeminfo, settings_t settings, patches_t *patches, cellsinfo_t *cellsinfo,
+                   grid_t *grid);
+void patches_free(patches_t *patches);
+void patches_env2cellsinit(systeminfo_t systeminfo, settings_t settings, patches_t *patches, grid_t *grid,
+                           environment_t **environment);
+void patches_env2cellswait(systeminfo_t systeminfo, settings_t settings, patches_t *patches,
+                           cellsinfo_t *cellsinfo, grid_t *grid, cellenvdata_t ***cellenvdata);
+void patches_cells2envwait(systeminfo_t systeminfo, settings_t settings, patches_t *patches, grid_t *grid,
+                           environment_t **environment);
+void patches_cells2envinit(systeminfo_t systeminfo, settings_t settings, patches_t *patches,
+                           cellsinfo_t *cellsinfo, grid_t *grid);
+
+/* ---- src/fieldgradient.h ---- */
+void fieldgradient(systeminfo_t systeminfo, settings_t settings, environment_t **environment, grid_t *grid);
+
+/* ---- configuration of the shim (no counterpart in the reference) ---- */
+/* 0 = FP64 drop-in (default, or TM_PRECISION=fp64), 1 = mixed (TM_PRECISION=mixed) */
+void tm_compat_set_precision(int precision);
+/* HIP device of the implicit context (default 0, or TM_DEVICE) */
+void tm_compat_set_device(int device);
+/* 1 (default, or TM_COMPAT_HOST_OCTREE=1): octbuild fills cellsinfo->octree with
+ * the PR-octree; 0: leaves cellsinfo->octree NULL for callers that never read it */
+void tm_compat_set_host_octree(int enable);
+/* the implicit context (a tm_ctx*, see timothy_hip.h) */
+void *tm_compat_context(void);
+/* release the implicit context */
+void tm_compat_shutdown(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,581 @@
+/*
+ * tm_compat.c -- the reference's octree / potential / patches / fieldgradient
+ * C API on top of the native C ABI (timothy_hip.h).
+ *
+ * Reference functions served (mcytu/timothy):
+ *   octbuild, octfree                    src/octree.c:145-207, 321-327
+ *   octcomputebox, octcomputeboxr        src/octree.c:214-266
+ *   octlocateregion                      src/octree.c:271-316
+ *   octheapinit/push/pop/free            src/octree.c:332-371
+ *   computepotential, computegradient    src/potential.c:87-138, 209-259
+ *   computeremotepotential/gradient      src/potential.c:144-188, 265-310 (one rank: no halo)
+ *   patches_*                            src/patches.c:47-657
+ *   fieldgradient                        src/fieldgradient.c:256-277
+ *
+ * One implicit device context serves the calls; it is created on first use
+ * and is not thread-safe (the reference calls these from one host thread).
+ * Failures go through the host program's terminate() when it defines one
+ * (the reference's utils.c), else print and abort.
+ */
+#include <float.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "timothy_compat.h"
+#include "timothy_hip.h"
+
+double3dv_t affShift;
+double affScale;
+
+/* the host program's terminate (src/utils.c:103-109), when it has one */
+extern void terminate(systeminfo_t systeminfo, char *msg, char *file, int line) __attribute__((weak));
+
+#define MAXVALUE 536870912.0 /* powf(2,29), src/global.h:67 */
+#define ROOT_LEVEL 29
+
+static struct {
+  tm_ctx *ctx;
+  int device;
+  int precision;
+  int host_octree;
+  int configured;
+  /* snapshot of the last octbuild */
+  const cellsinfo_t *built_for;
+  int64_t n;
+  int have_forces;
+  double *h_x, *h_y, *h_z, *h_size, *h_out;
+  float *h_young;
+  int32_t *h_ctype, *h_phase;
+  int64_t h_cap;
+  /* grid transfer buffers */
+  double *d_prod, *d_field, *d_grad, *d_interp;
+  int64_t grid_cap, interp_cap;
+  int deposited;
+  /* per-field copies for the interpolation */
+  double **d_envf, **d_envg;
+  int env_nf, env_ready;
+  int64_t env_nodes;
+} S;
+
+static void free_env(void);
+
+static void fail(systeminfo_t si, const char *msg, const char *file, int line) {
+  if (terminate) terminate(si, (char *)msg, (char *)file, line);
+  fprintf(stderr, "error: %s (%s, line %d)\n", msg, file, line);
+  fflush(stderr);
+  abort();
+}
+
+#define FAIL(si, msg) fail((si), (msg), __FILE__, __LINE__)
+
+static void check(systeminfo_t si, tm_status s, const char *what) {
+  if (s == TM_OK) return;
+  char buf[512];
+  snprintf(buf, sizeof(buf), "%s: %s %s", what, tm_status_name(s), S.ctx ? tm_last_error(S.ctx) : "");
+  FAIL(si, buf);
+}
+
+static void configure(void) {
+  if (S.configured) return;
+  const char *p = getenv("TM_PRECISION");
+  S.precision = (p && !strcmp(p, "mixed")) ? 1 : 0;
+  const char *d = getenv("TM_DEVICE");
+  S.device = d ? atoi(d) : 0;
+  const char *o = getenv("TM_COMPAT_HOST_OCTREE");
+  S.host_octree = o ? atoi(o) != 0 : 1;
+  S.configured = 1;
+}
+
+static tm_ctx *ctx_for(systeminfo_t si) {
+  configure();
+  if (si.size > 1) FAIL(si, "the MI355X cell-mechanics path serves one MPI rank (systeminfo.size == 1)");
+  if (!S.ctx) check(si, tm_create(&S.ctx, S.device), "tm_create");
+  return S.ctx;
+}
+
+void tm_compat_set_precision(int precision) {
+  configure();
+  S.precision = precision ? 1 : 0;
+}
+void tm_compat_set_device(int device) {
+  configure();
+  S.device = device;
+}
+void tm_compat_set_host_octree(int enable) {
+  configure();
+  S.host_octree = enable ? 1 : 0;
+}
+void *tm_compat_context(void) { return S.ctx; }
+
+static void free_host(void) {
+  free(S.h_x);
+  free(S.h_y);
+  free(S.h_z);
+  free(S.h_size);
+  free(S.h_out);
+  free(S.h_young);
+  free(S.h_ctype);
+  free(S.h_phase);
+  S.h_x = S.h_y = S.h_z = S.h_size = S.h_out = NULL;
+  S.h_young = NULL;
+  S.h_ctype = S.h_phase = NULL;
+  S.h_cap = 0;
+}
+
+void tm_compat_shutdown(void) {
+  if (S.ctx) {
+    free_env();
+    if (S.d_prod) tm_free(S.ctx, S.d_prod);
+    if (S.d_field) tm_free(S.ctx, S.d_field);
+    if (S.d_grad) tm_free(S.ctx, S.d_grad);
+    if (S.d_interp) tm_free(S.ctx, S.d_interp);
+    tm_destroy(S.ctx);
+  }
+  S.d_prod = S.d_field = S.d_grad = S.d_interp = NULL;
+  S.grid_cap = S.interp_cap = 0;
+  S.ctx = NULL;
+  S.built_for = NULL;
+  free_host();
+}
+
+static void ensure_host(systeminfo_t si, int64_t n) {
+  if (n <= S.h_cap) return;
+  free_host();
+  S.h_x = malloc(sizeof(double) * n);
+  S.h_y = malloc(sizeof(double) * n);
+  S.h_z = malloc(sizeof(double) * n);
+  S.h_size = malloc(sizeof(double) * n);
+  S.h_out = malloc(sizeof(double) * 6 * n);
+  S.h_young = malloc(sizeof(float) * n);
+  S.h_ctype = malloc(sizeof(int32_t) * n);
+  S.h_phase = malloc(sizeof(int32_t) * n);
+  if (!S.h_x || !S.h_y || !S.h_z || !S.h_size || !S.h_out || !S.h_young || !S.h_ctype || !S.h_phase)
+    FAIL(si, "cannot allocate host staging");
+  S.h_cap = n;
+}
+
+/* ------------------------------------------------------------------ */
+/* octree                                                              */
+/* ------------------------------------------------------------------ */
+
+void octbuild(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype) {
+  const int64_t n = (int64_t)ci->localcount.n;
+  if (n == 0) return; /* src/octree.c:152-153 */
+  tm_ctx *c = ctx_for(si);
+  ensure_host(si, n);
+  int maxt = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const celldata_t *cd = &ci->cells[i];
+    S.h_x[i] = cd->x;
+    S.h_y[i] = cd->y;
+    S.h_z[i] = cd->z;
+    S.h_size[i] = cd->size;
+    S.h_young[i] = cd->young;
+    S.h_ctype[i] = cd->ctype;
+    S.h_phase[i] = cd->phase;
+    if (cd->ctype > maxt) maxt = cd->ctype;
+    if (cd->ctype < 0) FAIL(si, "negative cell type");
+  }
+  int ntypes = maxt + 1;
+  if (ntypes > 16) FAIL(si, "at most 16 cell types");
+  tm_celltype t[16];
+  for (int k = 0; k < ntypes; k++) {
+    t[k].size = celltype[k].size;
+    t[k].h = celltype[k].h;
+    t[k].h2 = celltype[k].h2;
+    t[k].h3 = celltype[k].h3;
+    t[k].h4 = celltype[k].h4;
+  }
+  int dim = ci->dimension == 2 ? 2 : 3;
+  check(si, tm_set_celltypes(c, t, ntypes, dim), "tm_set_celltypes");
+  check(si, tm_upload(c, n, 1, NULL, S.h_x, S.h_y, S.h_z, S.h_size, S.h_young, S.h_ctype, S.h_phase), "tm_upload");
+  check(si, tm_build(c), "octbuild");
+  double sh[3], sc;
+  check(si, tm_get_affine(c, 0, sh, &sc), "tm_get_affine");
+  affShift.x = sh[0];
+  affShift.y = sh[1];
+  affShift.z = sh[2];
+  affScale = sc;
+  S.built_for = ci;
+  S.n = n;
+  S.have_forces = 0;
+  S.deposited = 0;
+  ci->octree = NULL;
+  ci->octsize = 0;
+  ci->octmaxsize = 0;
+  if (S.host_octree) {
+    int64_t cnt = 0;
+    check(si, tm_get_octree(c, 0, NULL, &cnt), "tm_get_octree");
+    octnode_t *o = (octnode_t *)malloc(sizeof(octnode_t) * (size_t)cnt);
+    if (!o) FAIL(si, "cannot allocate cellsinfo->octree");
+    check(si, tm_get_octree(c, 0, o, &cnt), "tm_get_octree");
+    ci->octree = o;
+    ci->octsize = cnt;
+    ci->octmaxsize = cnt;
+  }
+}
+
+void octfree(cellsinfo_t *ci) {
+  if (ci->localcount.n == 0) return; /* src/octree.c:323 */
+  free(ci->octree);
+  ci->octree = NULL;
+  if (S.built_for == ci) S.built_for = NULL;
+}
+
+/* src/octree.c:246-266 */
+void octcomputebox(int64_t c, uint3dv_t *mn, uint3dv_t *mx, cellsinfo_t ci, celltype_t *celltype) {
+  const celldata_t *cd = &ci.cells[c];
+  double h = celltype[cd->ctype].h;
+  mn->x = (unsigned int)(((cd->x - h - affShift.x) / affScale) * MAXVALUE);
+  mn->y = (unsigned int)(((cd->y - h - affShift.y) / affScale) * MAXVALUE);
+  mn->z = (unsigned int)(((cd->z - h - affShift.z) / affScale) * MAXVALUE);
+  mx->x = (unsigned int)(((cd->x + h - affShift.x) / affScale) * MAXVALUE);
+  mx->y = (unsigned int)(((cd->y + h - affShift.y) / affScale) * MAXVALUE);
+  mx->z = (unsigned int)(((cd->z + h - affShift.z) / affScale) * MAXVALUE);
+}
+
+/* src/octree.c:214-241: halo cells, box cropped to the unit cube */
+void octcomputeboxr(int64_t c, uint3dv_t *mn, uint3dv_t *mx, cellcommdata_t cc, celltype_t *celltype) {
+  const cellindata_t *cd = &cc.recvcellindata[c];
+  double h = celltype[cd->ctype].h;
+  double lo[3] = {(cd->x - h - affShift.x) / affScale, (cd->y - h - affShift.y) / affScale,
+                  (cd->z - h - affShift.z) / affScale};
+  double hi[3] = {(cd->x + h - affShift.x) / affScale, (cd->y + h - affShift.y) / affScale,
+                  (cd->z + h - affShift.z) / affScale};
+  for (int a = 0; a < 3; a++) {
+    lo[a] = lo[a] < 0.0 ? 0.0 : lo[a];
+    hi[a] = hi[a] > 1.0 ? 1.0 : hi[a];
+  }
+  mn->x = (unsigned int)(lo[0] * MAXVALUE);
+  mn->y = (unsigned int)(lo[1] * MAXVALUE);
+  mn->z = (unsigned int)(lo[2] * MAXVALUE);
+  mx->x = (unsigned int)(hi[0] * MAXVALUE);
+  mx->y = (unsigned int)(hi[1] * MAXVALUE);
+  mx->z = (unsigned int)(hi[2] * MAXVALUE);
+}
+
+static inline unsigned child_index(unsigned x, unsigned y, unsigned z, unsigned bit) {
+  return ((x >> bit) & 1u) + 2u * ((y >> bit) & 1u) + 4u * ((z >> bit) & 1u);
+}
+
+/* src/octree.c:271-316: smallest node containing the box.  A missing child
+ * ends the descent at its parent (the reference reads octree[-1] there). */
+int octlocateregion(uint3dv_t mn, uint3dv_t mx, cellsinfo_t ci) {
+  unsigned dx = mn.x ^ mx.x, dy = mn.y ^ mx.y, dz = mn.z ^ mx.z;
+  unsigned l1 = ROOT_LEVEL, l2 = ROOT_LEVEL, lmin = ROOT_LEVEL;
+  while (!(dx & (1u << l1)) && l1) l1--;
+  while (!(dy & (1u << l2)) && (l2 > l1)) l2--;
+  while (!(dz & (1u << lmin)) && (lmin > l2)) lmin--;
+  lmin++;
+  unsigned level = ROOT_LEVEL - 1;
+  unsigned steps = level - lmin + 1;
+  int64_t idx = 0, parent = 0;
+  while (steps--) {
+    unsigned ci_ = child_index(mn.x, mn.y, mn.z, level);
+    level--;
+    parent = idx;
+    idx = ci.octree[idx].child[ci_];
+    if (idx == -1) break;
+    if (ci.octree[idx].data != -2) break;
+  }
+  if (idx == -1) idx = parent;
+  return (int)idx;
+}
+
+/* src/octree.c:332-371 */
+void octheapinit(systeminfo_t si, octheap_t *h) {
+  h->size = 64;
+  h->count = 0;
+  if (!(h->data = malloc(sizeof(int) * h->size))) FAIL(si, "cannot allocate ttheap->data");
+}
+void octheappush(systeminfo_t si, octheap_t *h, int idx) {
+  if (h->count == h->size) {
+    h->size += 64;
+    int *d = realloc(h->data, sizeof(int) * h->size);
+    if (!d) FAIL(si, "cannot reallocate ttheap->data");
+    h->data = d;
+  }
+  h->data[h->count++] = idx;
+}
+int octheappop(octheap_t *h) { return h->data[--h->count]; }
+void octheapfree(octheap_t *h) { free(h->data); }
+
+/* ------------------------------------------------------------------ */
+/* potential                                                           */
+/* ------------------------------------------------------------------ */
+
+static void require_build(systeminfo_t si, const cellsinfo_t *ci) {
+  if (S.built_for != ci || S.n != (int64_t)ci->localcount.n)
+    FAIL(si, "computepotential/computegradient need octbuild on the same cells first");
+}
+
+static void run_pairs(systeminfo_t si, cellsinfo_t *ci) {
+  tm_ctx *c = S.ctx;
+  const int64_t n = S.n;
+  check(si, tm_neighbors(c), "neighbour lists");
+  check(si, tm_sph(c, S.precision ? TM_MIXED : TM_FP64), "pair sums");
+  double *o = S.h_out;
+  check(si, tm_get_results(c, o, o + n, o + 2 * n, o + 3 * n, o + 4 * n, o + 5 * n), "results");
+  S.have_forces = 1;
+  (void)ci;
+}
+
+/* src/potential.c:87-138 */
+void computepotential(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype, cellcommdata_t cc) {
+  if (ci->localcount.n <= 1) return; /* src/potential.c:89 */
+  ctx_for(si);
+  require_build(si, ci);
+  run_pairs(si, ci);
+  const int64_t n = S.n;
+  const double *o = S.h_out;
+  for (int64_t i = 0; i < n; i++) {
+    ci->cells[i].density = o[i];
+    ci->cells[i].v = o[n + i];
+    ci->cells[i].mindist = o[2 * n + i];
+  }
+  (void)celltype;
+  (void)cc;
+}
+
+/* src/potential.c:209-259 */
+void computegradient(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype, cellcommdata_t cc) {
+  if (ci->localcount.n <= 1) return; /* src/potential.c:212 */
+  ctx_for(si);
+  require_build(si, ci);
+  if (!S.have_forces) run_pairs(si, ci);
+  const int64_t n = S.n;
+  const double *o = S.h_out;
+  for (int64_t i = 0; i < n; i++) {
+    ci->forces[i].x = o[3 * n + i];
+    ci->forces[i].y = o[4 * n + i];
+    ci->forces[i].z = o[5 * n + i];
+  }
+  (void)celltype;
+  (void)cc;
+}
+
+/* src/potential.c:144-188 and 265-310: contributions of halo cells, which
+ * one rank does not have */
+void computeremotepotential(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype, cellcommdata_t cc) {
+  if (cc.numimp <= 0) return;
+  FAIL(si, "halo cells (numimp > 0) need more than one rank; not served on one GPU");
+  (void)ci;
+  (void)celltype;
+}
+void computeremotegradient(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype, cellcommdata_t cc) {
+  if (cc.numimp <= 0) return;
+  FAIL(si, "halo cells (numimp > 0) need more than one rank; not served on one GPU");
+  (void)ci;
+  (void)celltype;
+}
+
+/* ------------------------------------------------------------------ */
+/* patches: cells <-> grid                                             */
+/* ------------------------------------------------------------------ */
+
+static tm_grid grid_of(const grid_t *g, int rank) {
+  tm_grid t;
+  t.lower[0] = g->lowercorner.x;
+  t.lower[1] = g->lowercorner.y;
+  t.lower[2] = g->lowercorner.z;
+  t.resolution = g->resolution;
+  t.loweridx[0] = g->loweridx[rank].x;
+  t.loweridx[1] = g->loweridx[rank].y;
+  t.loweridx[2] = g->loweridx[rank].z;
+  t.upperidx[0] = g->upperidx[rank].x;
+  t.upperidx[1] = g->upperidx[rank].y;
+  t.upperidx[2] = g->upperidx[rank].z;
+  t.localsize[0] = g->localsize.x;
+  t.localsize[1] = g->localsize.y;
+  t.localsize[2] = g->localsize.z;
+  return t;
+}
+
+static int64_t grid_nodes(const grid_t *g) { return g->localsize.x * g->localsize.y * g->localsize.z; }
+
+static void ensure_grid(systeminfo_t si, int64_t nodes) {
+  if (nodes <= S.grid_cap) return;
+  if (S.d_prod) tm_free(S.ctx, S.d_prod);
+  if (S.d_field) tm_free(S.ctx, S.d_field);
+  if (S.d_grad) tm_free(S.ctx, S.d_grad);
+  S.d_prod = S.d_field = S.d_grad = NULL;
+  check(si, tm_malloc(S.ctx, (void **)&S.d_prod, sizeof(double) * nodes), "grid buffer");
+  check(si, tm_malloc(S.ctx, (void **)&S.d_field, sizeof(double) * nodes), "grid buffer");
+  check(si, tm_malloc(S.ctx, (void **)&S.d_grad, sizeof(double) * 3 * nodes), "grid buffer");
+  S.grid_cap = nodes;
+}
+
+/* src/patches.c:47-148, one rank: the patch is the bounding box of the
+ * in-range corner indices of all cells */
+void patches_alloc(systeminfo_t si, settings_t st, patches_t *pt, cellsinfo_t *ci, grid_t *grid) {
+  if (si.size > 1) FAIL(si, "the MI355X cell-mechanics path serves one MPI rank");
+  memset(pt, 0, sizeof(*pt));
+  pt->data = calloc(1, sizeof(double *));
+  pt->intersect = calloc(1, sizeof(int));
+  pt->lowercorner = calloc(1, sizeof(int643dv_t));
+  pt->uppercorner = calloc(1, sizeof(int643dv_t));
+  pt->lowercornerR = calloc(1, sizeof(int643dv_t));
+  pt->uppercornerR = calloc(1, sizeof(int643dv_t));
+  pt->size = calloc(1, sizeof(int643dv_t));
+  if (!pt->data || !pt->intersect || !pt->lowercorner || !pt->uppercorner || !pt->lowercornerR ||
+      !pt->uppercornerR || !pt->size)
+    FAIL(si, "cannot allocate patches");
+  int643dv_t lo = {INT64_MAX, INT64_MAX, ci->dimension == 3 ? INT64_MAX : 0};
+  int643dv_t hi = {INT64_MIN, INT64_MIN, ci->dimension == 3 ? INT64_MIN : 0};
+  const int643dv_t L = grid->loweridx[0], U = grid->upperidx[0];
+  for (uint64_t c = 0; c < ci->localcount.n; c++) {
+    int643dv_t k;
+    k.x = (int64_t)((ci->cells[c].x - grid->lowercorner.x) / grid->resolution);
+    k.y = (int64_t)((ci->cells[c].y - grid->lowercorner.y) / grid->resolution);
+    k.z = (int64_t)((ci->cells[c].z - grid->lowercorner.z) / grid->resolution);
+    for (int ax = 0; ax < 2; ax++)
+      for (int ay = 0; ay < 2; ay++)
+        for (int az = 0; az < 2; az++) {
+          if (!(k.x + ax >= L.x && k.y + ay >= L.y && k.z + az >= L.z && k.x + ax <= U.x && k.y + ay <= U.y &&
+                k.z + az <= U.z))
+            continue;
+          if (k.x + ax < lo.x) lo.x = k.x + ax;
+          if (k.y + ay < lo.y) lo.y = k.y + ay;
+          if (k.x + ax > hi.x) hi.x = k.x + ax;
+          if (k.y + ay > hi.y) hi.y = k.y + ay;
+          if (ci->dimension == 3) {
+            if (k.z + az < lo.z) lo.z = k.z + az;
+            if (k.z + az > hi.z) hi.z = k.z + az;
+          }
+          pt->intersect[0] = 1;
+        }
+  }
+  pt->lowercorner[0] = lo;
+  pt->uppercorner[0] = hi;
+  if (pt->intersect[0]) {
+    pt->size[0].x = hi.x - lo.x + 1;
+    pt->size[0].y = hi.y - lo.y + 1;
+    pt->size[0].z = ci->dimension == 3 ? hi.z - lo.z + 1 : 1;
+  }
+  (void)st;
+}
+
+/* src/patches.c:156-234 + 242-286: CIC deposit of field 0 on the device */
+void patches_cells2envinit(systeminfo_t si, settings_t st, patches_t *pt, cellsinfo_t *ci, grid_t *grid) {
+  if (st.numberoffields == 0) return;
+  ctx_for(si);
+  require_build(si, ci);
+  ensure_grid(si, grid_nodes(grid));
+  tm_grid g = grid_of(grid, si.rank);
+  check(si, tm_cic_deposit(S.ctx, &g, 0, S.d_prod), "CIC deposit");
+  pt->receiver = calloc(1, sizeof(int));
+  pt->sender = calloc(1, sizeof(int));
+  if (!pt->receiver || !pt->sender) FAIL(si, "cannot allocate patches");
+  pt->receiver[0] = pt->intersect[0];
+  pt->sender[0] = pt->intersect[0];
+  pt->lowercornerR[0] = pt->lowercorner[0];
+  pt->uppercornerR[0] = pt->uppercorner[0];
+  S.deposited = 1;
+}
+
+/* src/patches.c:292-367: production[f] = 0, then field 0 gets the deposit */
+void patches_cells2envwait(systeminfo_t si, settings_t st, patches_t *pt, grid_t *grid, environment_t **env) {
+  if (st.numberoffields == 0) return;
+  ctx_for(si);
+  if (!S.deposited) FAIL(si, "patches_cells2envwait without patches_cells2envinit");
+  const int64_t nodes = grid_nodes(grid);
+  for (int f = 1; f < st.numberoffields; f++) memset((*env)[f].production, 0, sizeof(double) * nodes);
+  check(si, tm_memcpy_dtoh(S.ctx, (*env)[0].production, S.d_prod, sizeof(double) * nodes), "production");
+  (void)pt;
+}
+
+static void free_env(void) {
+  for (int f = 0; f < S.env_nf; f++) {
+    if (S.d_envf[f]) tm_free(S.ctx, S.d_envf[f]);
+    if (S.d_envg[f]) tm_free(S.ctx, S.d_envg[f]);
+  }
+  free(S.d_envf);
+  free(S.d_envg);
+  S.d_envf = S.d_envg = NULL;
+  S.env_nf = 0;
+  S.env_nodes = 0;
+}
+
+/* src/patches.c:380-505: copies the fields and their gradients the
+ * interpolation reads (the reference packs them into commbuff here) */
+void patches_env2cellsinit(systeminfo_t si, settings_t st, patches_t *pt, grid_t *grid, environment_t **env) {
+  if (st.numberoffields == 0) return;
+  ctx_for(si);
+  const int64_t nodes = grid_nodes(grid);
+  const int nf = st.numberoffields;
+  if (nf != S.env_nf || nodes > S.env_nodes) {
+    free_env();
+    S.d_envf = calloc(nf, sizeof(double *));
+    S.d_envg = calloc(nf, sizeof(double *));
+    if (!S.d_envf || !S.d_envg) FAIL(si, "cannot allocate field copies");
+    S.env_nf = nf;
+    for (int f = 0; f < nf; f++) {
+      check(si, tm_malloc(S.ctx, (void **)&S.d_envf[f], sizeof(double) * nodes), "field copy");
+      check(si, tm_malloc(S.ctx, (void **)&S.d_envg[f], sizeof(double) * 3 * nodes), "gradient copy");
+    }
+    S.env_nodes = nodes;
+  }
+  for (int f = 0; f < nf; f++) {
+    check(si, tm_memcpy_htod(S.ctx, S.d_envf[f], (*env)[f].data, sizeof(double) * nodes), "field upload");
+    check(si, tm_memcpy_htod(S.ctx, S.d_envg[f], (*env)[f].gradient, sizeof(double) * 3 * nodes),
+          "gradient upload");
+  }
+  S.env_ready = 1;
+  (void)pt;
+}
+
+/* src/patches.c:512-593: cellenvdata[f][c] = CIC interpolation of field f and
+ * its gradient; cellenvdata is allocated here and freed by the caller
+ * (step_clean, src/step.c:102-108) */
+void patches_env2cellswait(systeminfo_t si, settings_t st, patches_t *pt, cellsinfo_t *ci, grid_t *grid,
+                           cellenvdata_t ***ced) {
+  if (st.numberoffields == 0) return;
+  ctx_for(si);
+  require_build(si, ci);
+  if (!S.env_ready || S.env_nf != st.numberoffields) FAIL(si, "patches_env2cellswait without patches_env2cellsinit");
+  const int64_t n = (int64_t)ci->localcount.n;
+  if (n > S.interp_cap) {
+    if (S.d_interp) tm_free(S.ctx, S.d_interp);
+    S.d_interp = NULL;
+    check(si, tm_malloc(S.ctx, (void **)&S.d_interp, sizeof(double) * 4 * n), "interpolation buffer");
+    S.interp_cap = n;
+  }
+  if (!((*ced) = (cellenvdata_t **)calloc(st.numberoffields, sizeof(cellenvdata_t *))))
+    FAIL(si, "cannot allocate cellenvdata");
+  tm_grid g = grid_of(grid, si.rank);
+  for (int f = 0; f < st.numberoffields; f++) {
+    if (!((*ced)[f] = (cellenvdata_t *)calloc(n, sizeof(cellenvdata_t)))) FAIL(si, "cannot allocate cellenvdata[f]");
+    check(si, tm_cic_interp(S.ctx, &g, S.d_envf[f], S.d_envg[f], S.d_interp), "CIC interpolation");
+    check(si, tm_memcpy_dtoh(S.ctx, (*ced)[f], S.d_interp, sizeof(cellenvdata_t) * n), "cellenvdata");
+  }
+  S.env_ready = 0;
+  (void)pt;
+}
+
+/* src/fieldgradient.c:149-277 (no neighbouring ranks: one-sided faces) */
+void fieldgradient(systeminfo_t si, settings_t st, environment_t **env, grid_t *grid) {
+  if (st.numberoffields == 0) return;
+  ctx_for(si);
+  const int64_t nodes = grid_nodes(grid);
+  ensure_grid(si, nodes);
+  tm_grid g = grid_of(grid, si.rank);
+  for (int f = 0; f < st.numberoffields; f++) {
+    check(si, tm_memcpy_htod(S.ctx, S.d_field, (*env)[f].data, sizeof(double) * nodes), "field upload");
+    check(si, tm_field_gradient(S.ctx, &g, S.d_field, S.d_grad), "field gradient");
+    check(si, tm_memcpy_dtoh(S.ctx, (*env)[f].gradient, S.d_grad, sizeof(double) * 3 * nodes), "gradient");
+  }
+}
+
+void patches_free(patches_t *pt) {
+  free(pt->receiver);
+  free(pt->sender);
+  free(pt->intersect);
+  free(pt->lowercorner);
+  free(pt->uppercorner);
+  free(pt->lowercornerR);
+  free(pt->uppercornerR);
+  free(pt->size);
+  free(pt->data);
+  memset(pt, 0, sizeof(*pt));
+}

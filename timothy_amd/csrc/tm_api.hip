@@ -619,9 +619,11 @@ tm_status tm_memset(tm_ctx *c, void *dst, int value, size_t bytes) {
 
 tm_status tm_time_stage(tm_ctx *c, tm_stage s, tm_precision p, int32_t warmup, int32_t iters, double *ms) {
   if (!c || !ms || iters < 1 || warmup < 0) return TM_ERR_INVALID;
-  if (s == TM_STAGE_NEIGHBORS || s == TM_STAGE_SPH || s == TM_STAGE_STEP) {
-    if (c->stage < 3) return tm_fail(c, TM_ERR_STATE, "run tm_neighbors once before timing");
-  }
+  // the neighbour lists must have been sized by one checked tm_neighbors call
+  if (s == TM_STAGE_NEIGHBORS && (c->stage < 2 || c->nbr_alloc == 0))
+    return tm_fail(c, TM_ERR_STATE, "run tm_neighbors once before timing");
+  if ((s == TM_STAGE_SPH || s == TM_STAGE_STEP) && c->stage < 3)
+    return tm_fail(c, TM_ERR_STATE, "run tm_neighbors once before timing");
   for (int i = 0; i < warmup; ++i) TM_TRY(run_stage(c, s, p));
   TM_HIP(c, hipEventRecord(c->ev0, c->stream));
   for (int i = 0; i < iters; ++i) TM_TRY(run_stage(c, s, p));
