@@ -180,6 +180,9 @@ tm_status tm_free(tm_ctx *ctx, void *ptr);
 tm_status tm_memcpy_htod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 tm_status tm_memcpy_dtoh(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 tm_status tm_memset(tm_ctx *ctx, void *dst, int value, size_t bytes);
+/* device-to-device copy on the context's stream (also the known-size
+ * reference transfer used to calibrate memory-traffic counters) */
+tm_status tm_memcpy_dtod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* -------- measurement -------- */
 /* runs `warmup` untimed then `iters` timed launches of a stage on the

@@ -56,6 +56,8 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
   free_ptr(c->d_out);
   free_ptr(c->d_split);
   free_ptr(c->d_parent);
+  free_ptr(c->d_pos_o);
+  free_ptr(c->d_aux_o);
   free_ptr(c->d_nbr);
   c->nbr_alloc = 0;
   c->nbr_elems = 0;
@@ -72,7 +74,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
       {(void **)&c->d_qstar, 4},    {(void **)&c->d_dens, 8},     {(void **)&c->d_v, 8},
       {(void **)&c->d_mind, 8},     {(void **)&c->d_fx, 8},       {(void **)&c->d_fy, 8},
       {(void **)&c->d_fz, 8},       {(void **)&c->d_out, 48},     {(void **)&c->d_split, 4},
-      {(void **)&c->d_parent, 8},
+      {(void **)&c->d_parent, 8},   {(void **)&c->d_pos_o, 32},   {(void **)&c->d_aux_o, 16},
   };
   for (auto &a : list) {
     if (hipMalloc(a.p, a.bytes * (size_t)cap) != hipSuccess) {
@@ -134,6 +136,31 @@ tm_status set_cells(tm_ctx *c, int64_t n, int32_t ncol, const int64_t *off, cons
   c->ncol = ncol;
   c->col_off = o;
   TM_HIP(c, hipMemcpyAsync(c->d_col_off, o.data(), sizeof(int64_t) * (ncol + 1), hipMemcpyHostToDevice, c->stream));
+  // bounding-box chunks: each covers at most TM_BBOX_CHUNK cells of one colony
+  std::vector<int64_t> cs;
+  std::vector<int32_t> cc;
+  for (int k = 0; k < ncol; ++k)
+    for (int64_t s = o[k]; s < o[k + 1]; s += TM_BBOX_CHUNK) {
+      cs.push_back(s);
+      cc.push_back(k);
+    }
+  cs.push_back(n);
+  c->nchunks = (int64_t)cc.size();
+  if (c->nchunks + 1 > c->cap_chunks) {
+    free_ptr(c->d_chunk_start);
+    free_ptr(c->d_chunk_cid);
+    c->cap_chunks = 0;
+    if (hipMalloc((void **)&c->d_chunk_start, sizeof(int64_t) * (c->nchunks + 1)) != hipSuccess ||
+        hipMalloc((void **)&c->d_chunk_cid, sizeof(int32_t) * (c->nchunks + 1)) != hipSuccess)
+      return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed for bounding-box chunks");
+    c->cap_chunks = c->nchunks + 1;
+  }
+  TM_HIP(c, hipMemcpyAsync(c->d_chunk_start, cs.data(), sizeof(int64_t) * cs.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  if (c->nchunks > 0)
+    TM_HIP(c, hipMemcpyAsync(c->d_chunk_cid, cc.data(), sizeof(int32_t) * cc.size(), hipMemcpyHostToDevice,
+                             c->stream));
+  TM_HIP(c, hipStreamSynchronize(c->stream));
   if (device) {
     c->external = true;
     c->x = x;
@@ -302,6 +329,10 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_flags);
   free_ptr(c->d_split);
   free_ptr(c->d_parent);
+  free_ptr(c->d_pos_o);
+  free_ptr(c->d_aux_o);
+  free_ptr(c->d_chunk_start);
+  free_ptr(c->d_chunk_cid);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -365,14 +396,20 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     if (v < 1 || v > 4096) return tm_fail(c, TM_ERR_INVALID, "nbr_cap in [1, 4096]");
     c->nbr_cap = v;
   } else if (!strcmp(key, "sph_block")) {
-    if (v != 64 && v != 128 && v != 256 && v != 512) return tm_fail(c, TM_ERR_INVALID, "sph_block in {64,128,256,512}");
+    if (v != 64 && v != 128 && v != 256) return tm_fail(c, TM_ERR_INVALID, "sph_block in {64,128,256}");
     c->sph_block = (int)v;
+  } else if (!strcmp(key, "sph_unroll_fp64") || !strcmp(key, "sph_unroll_mixed")) {
+    if (v != 1 && v != 2 && v != 4 && v != 8) return tm_fail(c, TM_ERR_INVALID, "sph_unroll in {1,2,4,8}");
+    if (key[11] == 'f') c->sph_unroll_fp64 = (int)v;
+    else c->sph_unroll_mixed = (int)v;
   } else if (!strcmp(key, "nbr_block")) {
     if (v != 64 && v != 128 && v != 256) return tm_fail(c, TM_ERR_INVALID, "nbr_block in {64,128,256}");
     c->nbr_block = (int)v;
   } else if (!strcmp(key, "sph_lanes")) {
-    if (v != 1 && v != 4 && v != 8 && v != 16) return tm_fail(c, TM_ERR_INVALID, "sph_lanes in {1,4,8,16}");
+    if (v != 1 && v != 2 && v != 4 && v != 8) return tm_fail(c, TM_ERR_INVALID, "sph_lanes in {1,2,4,8}");
     c->sph_lanes = (int)v;
+  } else if (!strcmp(key, "sph_balance")) {
+    c->sph_balance = v ? 1 : 0;
   } else if (!strcmp(key, "mixed_recentre")) {
     c->mixed_recentre = v ? 1 : 0;
   } else {
@@ -385,8 +422,11 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   if (!c || !key || !v) return TM_ERR_INVALID;
   if (!strcmp(key, "nbr_cap")) *v = c->nbr_cap;
   else if (!strcmp(key, "sph_block")) *v = c->sph_block;
+  else if (!strcmp(key, "sph_unroll_fp64")) *v = c->sph_unroll_fp64;
+  else if (!strcmp(key, "sph_unroll_mixed")) *v = c->sph_unroll_mixed;
   else if (!strcmp(key, "nbr_block")) *v = c->nbr_block;
   else if (!strcmp(key, "sph_lanes")) *v = c->sph_lanes;
+  else if (!strcmp(key, "sph_balance")) *v = c->sph_balance;
   else if (!strcmp(key, "mixed_recentre")) *v = c->mixed_recentre;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
   else return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
@@ -532,12 +572,13 @@ tm_status tm_get_neighbors(tm_ctx *c, int64_t *offsets, int32_t *nbr, int64_t *t
   }
   if (total) *total = tot;
   if (nbr && offsets && n > 0) {
-    std::vector<uint32_t> slots((size_t)n * (size_t)cap);
-    TM_HIP(c, hipMemcpyAsync(slots.data(), c->d_nbr, 4 * (size_t)n * (size_t)cap, hipMemcpyDeviceToHost, c->stream));
+    const int64_t elems = tm_slot_elems(n, cap);
+    std::vector<uint32_t> slots((size_t)elems);
+    TM_HIP(c, hipMemcpyAsync(slots.data(), c->d_nbr, 4 * (size_t)elems, hipMemcpyDeviceToHost, c->stream));
     TM_HIP(c, hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < n; ++i) {
       int64_t o = offsets[p[i]];
-      for (uint32_t k = 0; k < cnt[i]; ++k) nbr[o + k] = p[slots[(size_t)k * n + i]];
+      for (uint32_t k = 0; k < cnt[i]; ++k) nbr[o + k] = p[slots[(size_t)tm_slot(i, k, cap)]];
     }
   }
   return TM_OK;
@@ -606,6 +647,13 @@ tm_status tm_memcpy_htod(tm_ctx *c, void *dst, const void *src, size_t bytes) {
 tm_status tm_memcpy_dtoh(tm_ctx *c, void *dst, const void *src, size_t bytes) {
   if (!c) return TM_ERR_INVALID;
   TM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  TM_HIP(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+tm_status tm_memcpy_dtod(tm_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c) return TM_ERR_INVALID;
+  TM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
   TM_HIP(c, hipStreamSynchronize(c->stream));
   return TM_OK;
 }
@@ -714,22 +762,36 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   }
   c->nbr_block = best_nb;
   TM_TRY(tm_run_neighbors(c, true));
-  // pair kernel workgroup
+  // pair kernel: workgroup size x neighbours fetched per trip
   int best_sb = c->sph_block;
+  int &unroll = (p == TM_MIXED) ? c->sph_unroll_mixed : c->sph_unroll_fp64;
+  int best_u = unroll, best_l = c->sph_lanes;
   best_t = 1e30;
-  for (int bs : {64, 128, 256, 512}) {
-    c->sph_block = bs;
-    double ms = 0;
-    TM_TRY(tm_time_stage(c, TM_STAGE_SPH, p, 2, 10, &ms));
-    snprintf(line, sizeof(line), "sph_block=%d %.4f ms\n", bs, ms);
-    rep += line;
-    if (ms < best_t) {
-      best_t = ms;
-      best_sb = bs;
+  const std::vector<int> lanes = (p == TM_MIXED) ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1};
+  for (int l : lanes) {
+    for (int bs : {64, 128, 256}) {
+      for (int u : {1, 2, 4, 8}) {
+        c->sph_block = bs;
+        unroll = u;
+        if (p == TM_MIXED) c->sph_lanes = l;
+        double ms = 0;
+        TM_TRY(tm_time_stage(c, TM_STAGE_SPH, p, 2, 10, &ms));
+        snprintf(line, sizeof(line), "lanes=%d sph_block=%d unroll=%d %.4f ms\n", l, bs, u, ms);
+        rep += line;
+        if (ms < best_t) {
+          best_t = ms;
+          best_sb = bs;
+          best_u = u;
+          best_l = l;
+        }
+      }
     }
   }
   c->sph_block = best_sb;
-  snprintf(line, sizeof(line), "chosen nbr_block=%d sph_block=%d\n", best_nb, best_sb);
+  unroll = best_u;
+  if (p == TM_MIXED) c->sph_lanes = best_l;
+  snprintf(line, sizeof(line), "chosen nbr_block=%d sph_block=%d unroll=%d lanes=%d (%s)\n", best_nb, best_sb,
+           best_u, p == TM_MIXED ? best_l : 1, p == TM_MIXED ? "mixed" : "fp64");
   rep += line;
   if (buf && buflen) {
     strncpy(buf, rep.c_str(), buflen - 1);

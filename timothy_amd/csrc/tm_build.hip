@@ -81,59 +81,55 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long 
   if (i < n) p[i] = ((i % per) < 3) ? v0 : v1;
 }
 
+// bounding box of x -+ (h_type + 0.01) per colony (src/octree.c:161-174): one
+// workgroup per chunk of one colony, reduced in registers and LDS, then six
+// atomics per workgroup on order-preserving integer images of the doubles
 __global__ __launch_bounds__(BT) void k_bbox(const double *__restrict__ x, const double *__restrict__ y,
                                              const double *__restrict__ z, const int32_t *__restrict__ ctype,
-                                             int64_t n, const int64_t *__restrict__ off, int ncol,
+                                             const int64_t *__restrict__ chunk_start,
+                                             const int32_t *__restrict__ chunk_cid,
                                              const TypeDev *__restrict__ types, int ntypes,
                                              unsigned long long *__restrict__ bbox, unsigned int *__restrict__ flags) {
-  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
-  bool valid = i < n;
-  int cid = 0;
-  unsigned long long k[6];
-  if (valid) {
-    cid = colony_of(off, ncol, i);
+  __shared__ unsigned long long red[BT / 64][6];
+  const int64_t s = chunk_start[blockIdx.x], e = chunk_start[blockIdx.x + 1];
+  unsigned long long k[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+  for (int64_t i = s + threadIdx.x; i < e; i += BT) {
     int t = ctype[i];
     if (t < 0 || t >= ntypes) {
       atomicOr(flags, (unsigned)TM_FLAG_CTYPE);
       t = 0;
     }
-    double e = types[t].h + 0.01;  // src/octree.c:167
-    double px = x[i], py = y[i], pz = z[i];
-    k[0] = ord_key(px - e);
-    k[1] = ord_key(py - e);
-    k[2] = ord_key(pz - e);
-    k[3] = ord_key(px + e);
-    k[4] = ord_key(py + e);
-    k[5] = ord_key(pz + e);
-  } else {
-    k[0] = k[1] = k[2] = ~0ull;
-    k[3] = k[4] = k[5] = 0ull;
+    const double ext = types[t].h + 0.01;  // src/octree.c:167
+    const double px = x[i], py = y[i], pz = z[i];
+    const unsigned long long v[6] = {ord_key(px - ext), ord_key(py - ext), ord_key(pz - ext),
+                                     ord_key(px + ext), ord_key(py + ext), ord_key(pz + ext)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      k[a] = v[a] < k[a] ? v[a] : k[a];
+      k[a + 3] = v[a + 3] > k[a + 3] ? v[a + 3] : k[a + 3];
+    }
   }
-  int cid0 = __shfl(cid, 0, 64);
-  unsigned long long act = __ballot(1);
-  bool uniform = __ballot(!valid || cid == cid0) == act;
-  if (uniform) {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
+  for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        unsigned long long t = __shfl_xor(k[a], o, 64);
-        k[a] = t < k[a] ? t : k[a];
-        unsigned long long u = __shfl_xor(k[a + 3], o, 64);
-        k[a + 3] = u > k[a + 3] ? u : k[a + 3];
-      }
+    for (int a = 0; a < 3; ++a) {
+      unsigned long long t = __shfl_xor(k[a], o, 64);
+      k[a] = t < k[a] ? t : k[a];
+      unsigned long long u = __shfl_xor(k[a + 3], o, 64);
+      k[a + 3] = u > k[a + 3] ? u : k[a + 3];
     }
-    if (__lane_id() == 0) {
-      unsigned long long *b = bbox + 6 * (int64_t)cid0;
-#pragma unroll
+  }
+  const int w = threadIdx.x >> 6;
+  if (__lane_id() == 0)
+    for (int a = 0; a < 6; ++a) red[w][a] = k[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < BT / 64; ++q)
       for (int a = 0; a < 3; ++a) {
-        atomicMin(b + a, k[a]);
-        atomicMax(b + 3 + a, k[a + 3]);
+        k[a] = red[q][a] < k[a] ? red[q][a] : k[a];
+        k[a + 3] = red[q][a + 3] > k[a + 3] ? red[q][a + 3] : k[a + 3];
       }
-    }
-  } else if (valid) {
-    unsigned long long *b = bbox + 6 * (int64_t)cid;
-#pragma unroll
+    unsigned long long *b = bbox + 6 * (int64_t)chunk_cid[blockIdx.x];
     for (int a = 0; a < 3; ++a) {
       atomicMin(b + a, k[a]);
       atomicMax(b + 3 + a, k[a + 3]);
@@ -157,44 +153,51 @@ __global__ void k_affine(const unsigned long long *__restrict__ bbox, int ncol, 
   aff[c] = make_double4(mn[0], mn[1], mn[2], s);
 }
 
+// Morton keys, and the cell inputs packed in reference order (one 32-byte and
+// one 16-byte record per cell) so the Morton gather below reads two records
+// per cell instead of seven scattered arrays.
 __global__ __launch_bounds__(BT) void k_keys(const double *__restrict__ x, const double *__restrict__ y,
-                                             const double *__restrict__ z, int64_t n,
-                                             const int64_t *__restrict__ off, int ncol, int cb,
+                                             const double *__restrict__ z, const double *__restrict__ size,
+                                             const float *__restrict__ young, const int32_t *__restrict__ ctype,
+                                             const int32_t *__restrict__ phase, int64_t n,
+                                             const int64_t *__restrict__ off, int ncol, int cb, int ntypes,
                                              const double4 *__restrict__ aff, uint64_t *__restrict__ key,
-                                             int32_t *__restrict__ val) {
+                                             int32_t *__restrict__ val, double4 *__restrict__ pos_o,
+                                             uint4 *__restrict__ aux_o) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n) return;
-  int cid = ncol > 1 ? colony_of(off, ncol, i) : 0;
-  double4 a = aff[cid];
-  uint32_t cx = loc_code(x[i], a.x, a.w), cy = loc_code(y[i], a.y, a.w), cz = loc_code(z[i], a.z, a.w);
+  const int cid = ncol > 1 ? colony_of(off, ncol, i) : 0;
+  const double4 a = aff[cid];
+  const double px = x[i], py = y[i], pz = z[i];
+  const uint32_t cx = loc_code(px, a.x, a.w), cy = loc_code(py, a.y, a.w), cz = loc_code(pz, a.z, a.w);
   uint64_t k = morton_prefix64(cx, cy, cz);
   if (cb > 0) k = ((uint64_t)cid << (64 - cb)) | (k >> cb);
   key[i] = k;
   val[i] = (int32_t)i;
+  pos_o[i] = make_double4(px, py, pz, size[i]);
+  const int32_t t = ctype[i];
+  aux_o[i] = make_uint4(__float_as_uint(young ? young[i] : 0.0f),
+                        (uint32_t)((t < 0 || t >= ntypes) ? 0 : t),  // out-of-range types were flagged by k_bbox
+                        (uint32_t)(phase ? phase[i] : 1), (uint32_t)cid);
 }
 
 __global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm, int64_t n,
-                                               const double *__restrict__ x, const double *__restrict__ y,
-                                               const double *__restrict__ z, const double *__restrict__ size,
-                                               const float *__restrict__ young, const int32_t *__restrict__ ctype,
-                                               const int32_t *__restrict__ phase, const int64_t *__restrict__ off,
-                                               int ncol, const double4 *__restrict__ aff, uint4 *__restrict__ code,
+                                               const double4 *__restrict__ pos_o, const uint4 *__restrict__ aux_o,
+                                               const double4 *__restrict__ aff, uint4 *__restrict__ code,
                                                double4 *__restrict__ pos, float *__restrict__ young_s,
-                                               int32_t *__restrict__ ctype_s, int32_t *__restrict__ phase_s,
-                                               int ntypes) {
+                                               int32_t *__restrict__ ctype_s, int32_t *__restrict__ phase_s) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n) return;
   int64_t o = perm[i];
   if (o < 0 || o >= n) o = 0;  // unreachable for a valid permutation; bounds the gathers
-  int cid = ncol > 1 ? colony_of(off, ncol, o) : 0;
-  double4 a = aff[cid];
-  double px = x[o], py = y[o], pz = z[o];
-  code[i] = make_uint4(loc_code(px, a.x, a.w), loc_code(py, a.y, a.w), loc_code(pz, a.z, a.w), (uint32_t)cid);
-  pos[i] = make_double4(px, py, pz, size[o]);
-  young_s[i] = young ? young[o] : 0.0f;
-  const int32_t t = ctype[o];
-  ctype_s[i] = (t < 0 || t >= ntypes) ? 0 : t;  // out-of-range types were flagged by k_bbox
-  phase_s[i] = phase ? phase[o] : 1;
+  const double4 P = pos_o[o];
+  const uint4 X = aux_o[o];
+  const double4 a = aff[X.w];
+  code[i] = make_uint4(loc_code(P.x, a.x, a.w), loc_code(P.y, a.y, a.w), loc_code(P.z, a.z, a.w), X.w);
+  pos[i] = P;
+  young_s[i] = __uint_as_float(X.x);
+  ctype_s[i] = (int32_t)X.y;
+  phase_s[i] = (int32_t)X.z;
 }
 
 // Cells whose 64-bit key prefixes tie are put in full 87-bit key order; the
@@ -395,20 +398,21 @@ tm_status tm_run_build(tm_ctx *c) {
   // bounding boxes and affine maps
   int64_t nb6 = 6 * (int64_t)c->ncol;
   k_fill_u64<<<tm_blocks(nb6, 256), 256, 0, st>>>(c->d_bbox, nb6, ~0ull, 0ull, 6);
-  k_bbox<<<tm_blocks(n, BT), BT, 0, st>>>(c->x, c->y, c->z, c->ctype, n, c->d_col_off, c->ncol, c->d_types,
-                                           c->ntypes, c->d_bbox, c->d_flags);
+  if (c->nchunks > 0)
+    k_bbox<<<(unsigned)c->nchunks, BT, 0, st>>>(c->x, c->y, c->z, c->ctype, c->d_chunk_start, c->d_chunk_cid,
+                                                 c->d_types, c->ntypes, c->d_bbox, c->d_flags);
   k_affine<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_bbox, c->ncol, c->d_aff);
   TM_HIP(c, hipGetLastError());
   // Morton keys and the sort
   int cb = 0;
   while ((1ll << cb) < c->ncol) ++cb;
-  k_keys<<<tm_blocks(n, BT), BT, 0, st>>>(c->x, c->y, c->z, n, c->d_col_off, c->ncol, cb, c->d_aff, c->d_key,
-                                           c->d_perm);
+  k_keys<<<tm_blocks(n, BT), BT, 0, st>>>(c->x, c->y, c->z, c->size, c->young, c->ctype, c->phase, n, c->d_col_off,
+                                           c->ncol, cb, c->ntypes, c->d_aff, c->d_key, c->d_perm, c->d_pos_o,
+                                           c->d_aux_o);
   TM_HIP(c, hipGetLastError());
   TM_TRY(tm_sort_pairs(c, c->d_key, c->d_perm, n));
-  k_gather<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_perm, n, c->x, c->y, c->z, c->size, c->young, c->ctype, c->phase,
-                                             c->d_col_off, c->ncol, c->d_aff, c->d_code, c->d_pos, c->d_young_s,
-                                             c->d_ctype_s, c->d_phase_s, c->ntypes);
+  k_gather<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_perm, n, c->d_pos_o, c->d_aux_o, c->d_aff, c->d_code, c->d_pos,
+                                             c->d_young_s, c->d_ctype_s, c->d_phase_s);
   k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_young_s,
                                               c->d_ctype_s, c->d_phase_s, c->d_flags);
   k_leaf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, n, c->d_span, c->d_flags);

@@ -63,7 +63,7 @@ __global__ void k_fill_f64(double *p, int64_t n) {
 __global__ __launch_bounds__(256) void k_deposit(const double4 *__restrict__ pos, const int32_t *__restrict__ phase_s,
                                                  int64_t n, GridDev g, int weight_mode,
                                                  double *__restrict__ prod) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = tm_xcd_block() * 256 + threadIdx.x;
   if (i >= n) return;
   const double4 P = pos[i];
   double wt;
@@ -95,7 +95,7 @@ template <typename F>
 __global__ __launch_bounds__(256) void k_interp(const double4 *__restrict__ pos, const int32_t *__restrict__ perm,
                                                 int64_t n, GridDev g, const F *__restrict__ field,
                                                 const F *__restrict__ grad, double *__restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t i = tm_xcd_block() * 256 + threadIdx.x;
   if (i >= n) return;
   const double4 P = pos[i];
   int64_t ci[3];

@@ -16,6 +16,7 @@
 #define TM_FLAG_ROOT 2
 #define TM_FLAG_CTYPE 4
 #define TM_FLAG_WALK 8
+#define TM_BBOX_CHUNK 8192
 
 // Per cell-type constants, derived on the host with the reference's own
 // expressions (src/inline.h:45-46,71-72) so the device reads identical values.
@@ -51,9 +52,12 @@ struct tm_ctx {
 
   // tunables
   int64_t nbr_cap = 48;
-  int sph_block = 256;
+  int sph_block = 128;
+  int sph_unroll_fp64 = 4;   // neighbours fetched per trip in the FP64 pair kernel
+  int sph_unroll_mixed = 8;  // the same for the mixed kernel
   int nbr_block = 256;
   int sph_lanes = 1;
+  int sph_balance = 1;  // FP64 kernel: rank cells by list length inside each workgroup
   int mixed_recentre = 0;
 
   // cells (reference order)
@@ -70,6 +74,13 @@ struct tm_ctx {
   int64_t own_cap = 0;
   int64_t *d_col_off = nullptr;
   int64_t cap_col = 0;
+  // bounding-box work split: chunks of at most TM_BBOX_CHUNK cells of one colony
+  int64_t nchunks = 0, cap_chunks = 0;
+  int64_t *d_chunk_start = nullptr;  // nchunks + 1
+  int32_t *d_chunk_cid = nullptr;
+  // inputs packed in reference order by the key kernel
+  double4 *d_pos_o = nullptr;  // x, y, z, size
+  uint4 *d_aux_o = nullptr;    // young bits, ctype, phase, colony
 
   // per colony
   double4 *d_aff = nullptr;              // shift.xyz, scale
@@ -189,3 +200,23 @@ __device__ __forceinline__ int tm_diffbit(uint4 a, uint4 b) {
 }
 
 inline unsigned tm_blocks(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// Neighbour-list layout: slot k of cell p.  The slots of each group of 64
+// consecutive cells are stored together, slot-major inside the group, so one
+// wave reads slot k of its 64 cells as one contiguous 256-byte line pair and
+// a group's whole list lies in one contiguous stretch of memory.
+__host__ __device__ __forceinline__ int64_t tm_slot(int64_t p, int64_t k, int64_t cap) {
+  return (((p >> 6) * cap + k) << 6) | (p & 63);
+}
+inline int64_t tm_slot_elems(int64_t n, int64_t cap) { return ((n + 63) >> 6) * cap * 64; }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2.  Cell-ordered kernels remap so that XCD x runs one
+// contiguous slice of the blocks, i.e. one compact stretch of the Morton
+// order, and its L2 holds that region's cells and tree nodes.  Bijective on
+// [0, gridDim.x).
+__device__ __forceinline__ int64_t tm_xcd_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t x = b & 7, j = b >> 3, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + j;
+}

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
                                                   uint32_t *__restrict__ nbr, uint32_t *__restrict__ count,
                                                   double *__restrict__ mindist, int32_t *__restrict__ qstar,
                                                   unsigned int *__restrict__ flags) {
-  int64_t p = (int64_t)blockIdx.x * BS + threadIdx.x;
+  int64_t p = tm_xcd_block() * BS + threadIdx.x;
   if (p >= n) return;
   const double4 P = pos[p];
   const uint32_t cid = code[p].w;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
         md2 = d2 < md2 ? d2 : md2;
         qs = (int32_t)q;
         if (d2 <= cut2) {
-          if (cnt < cap) nbr[(int64_t)cnt * n + p] = q;
+          if (cnt < cap) nbr[tm_slot(p, cnt, cap)] = q;
           ++cnt;
         }
       }
@@ -98,7 +98,7 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     return TM_OK;
   }
   for (int attempt = 0; attempt < 2; ++attempt) {
-    const int64_t need = n * c->nbr_cap;
+    const int64_t need = tm_slot_elems(n, c->nbr_cap);
     if (c->nbr_elems < need || !c->d_nbr) {
       if (c->d_nbr) (void)hipFree(c->d_nbr);
       c->d_nbr = nullptr;
