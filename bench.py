@@ -1,14 +1,22 @@
 """Benchmark: SPH density+force pair-interactions/s on one MI355X.
 
-Workload: one colony of 2,000,000 cells (radius 1.0, cutoff h = 2.0) drawn
-from a seeded Gaussian spheroid (sigma = 0.35 * 146.6, z axis * 0.7), held in
-HBM.  Untimed: Morton sort, radix tree, neighbour lists.  Timed: K launches of
-the SPH density+force stage (density, v and forces over the neighbour lists),
-mixed precision and plain FP64, each bracketed by HIP events on the library's
-stream and synchronised on both sides.  `value` is the mixed form's
-pair-interactions/s (P / t_sph, P = ordered in-cutoff pairs); the FP64 drop-in,
-the full-step time (sort + tree + neighbour lists + SPH + CIC deposit to a 256^3
-FP64 grid) and the mixed-vs-FP64 errors are reported beside it.
+Default workload (the metric's): one colony of 2,000,000 cells (radius 1.0,
+cutoff h = 2.0) drawn from a seeded Gaussian spheroid (sigma = 0.35 * 146.6,
+z axis * 0.7), held in HBM.  Untimed: Morton sort, radix tree, neighbour
+lists.  Timed: K launches of the SPH density+force stage (density, v and
+forces over the neighbour lists), mixed precision and plain FP64, each
+bracketed by HIP events on the library's stream and synchronised on both
+sides.  `value` is the mixed form's pair-interactions/s (P / t_sph, P =
+ordered in-cutoff pairs); the FP64 drop-in, the full-step time (sort + tree +
+neighbour lists + SPH + CIC deposit to a 256^3 FP64 grid) and the
+mixed-vs-FP64 errors are reported beside it.
+
+--workload batched: 256 independent colonies of 1,000..200,000 cells
+(log-uniform, ~9.6M cells), one launch per stage over all colonies, the step
+captured in a HIP graph and replayed; per-colony mixed-vs-FP64 errors.
+--workload large: 50,000,000 cells, 5% of radius 1.5; mixed SPH, CIC deposit
+of cell volume onto 512^3 FP64, FP32 field gradient and interpolation of a
+seeded smooth field on 512^3.
 
 roofline: algorithmic bytes of one SPH launch, B = 96 N + 4 P (per cell:
 x,y,z,size 32, ctype 4, list offset 8, young 4 read; density 8, force 24, v 8,
@@ -17,8 +25,9 @@ MI355X's 8 TB/s.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE /
 WRITE_SIZE counters when a committed summary exists (profiles/), else null.
 
 cpu_baseline: the CPU restatement of the reference path (pointer PR-octree,
-per-cell DFS walks with OpenMP dynamic,64) on this host's cores, same
-workload, the two walks timed (median of 3 after one warm-up).
+per-cell DFS walks with OpenMP dynamic,64) on this host's cores, the two
+walks timed (median of 3 after one warm-up); for the default workload on the
+full colony, for the others on a bounded sample named in the line.
 """
 from __future__ import annotations
 
@@ -34,6 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "SPH density+force pair-interactions/s, mixed vs FP64 plain, 1 MI355X"
 
 
 def parse():
@@ -41,12 +51,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=["single", "batched", "large"], default="single")
     ap.add_argument("--scale", type=float, default=1.0, help="cell-count scale (1.0 = the named workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0 = all)")
     ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
     ap.add_argument("--autotune", action="store_true")
     return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def traffic_from_profiles():
@@ -60,7 +75,8 @@ def traffic_from_profiles():
         return None
 
 
-def cpu_baseline(col, threads):
+def cpu_walks(col, threads):
+    """oracle octbuild + the two per-cell walks; median of 3 after a warm-up"""
     from oracle import oracle as O
 
     if threads > 0:
@@ -85,25 +101,92 @@ def cpu_baseline(col, threads):
     return nthr, t_build, tp, tg
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
-                          "n_gpus": a.gpus}))
-        sys.exit(2)
-    import timothy_amd as tm
-    from timothy_amd import generators as gen
+def gpu_pairs(tm, col):
+    with tm.Context(0) as c:
+        c.upload_colony(col)
+        c.step(tm.FP64)
+        return c.pair_count()[0]
 
-    col = gen.config(1, scale=a.scale)
+
+def copy_bandwidth(tm, ctx, mib=2048, iters=10):
+    """device-to-device copy rate (read + write bytes / s): the achievable HBM figure"""
+    nb = mib << 20
+    src = tm.DeviceArray(ctx, nb, np.uint8)
+    dst = tm.DeviceArray(ctx, nb, np.uint8)
+    src.zero()
+    f = tm.lib().tm_memcpy_dtod
+    ctx._check(f(ctx.h, dst.ptr, src.ptr, nb))
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        ctx._check(f(ctx.h, dst.ptr, src.ptr, nb))
+    dt = time.perf_counter() - t0
+    src.free()
+    dst.free()
+    return 2.0 * nb * iters / dt / 1e9
+
+
+def pair_abs_sum(col, off, nbr, chunk=1 << 21):
+    """per cell sum_q |f_pq| of the FP64 pair force (normalises the force error)"""
     n = col["n"]
-    ctx = tm.Context(0)
-    ctx.upload_colony(col)
-    ctx.step(tm.FP64)
-    P, maxnbr = ctx.pair_count()
-    ref = ctx.results()  # FP64 drop-in, bit-exact with the CPU path
-    if a.autotune:
-        print(ctx.autotune(tm.MIXED), file=sys.stderr)
+    h = np.array([t["h"] for t in col["types"]])
+    h4 = np.array([t["h4"] for t in col["types"]])
+    X = np.stack([col["x"], col["y"], col["z"]], 1)
+    out = np.zeros(n)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        cnt = np.diff(off[s:e + 1])
+        rows = np.repeat(np.arange(s, e), cnt)
+        q = nbr[off[s]:off[e]]
+        d = np.sqrt(((X[rows] - X[q]) ** 2).sum(1))
+        ct = col["ctype"][rows]
+        u = d / h[ct]
+        w = np.where(u <= 0.5, -2 * u + 3 * u * u, -(1 - u) ** 2)
+        out[s:e] = np.bincount(rows - s, np.abs(w) * 48 / (np.pi * h4[ct]), minlength=e - s)
+    return out
 
+
+def sph_errors(col, off, nbr, mix, ref):
+    sabs = pair_abs_sum(col, off, nbr)
+    has = sabs > 0
+    fe = np.abs(mix["force"] - ref["force"]).max(1)[has] / sabs[has]
+    fr = np.abs(mix["force"] - ref["force"]).max() / np.abs(ref["force"]).max()
+    pos = ref["density"] > 0
+    de = np.abs(mix["density"] - ref["density"])[pos] / ref["density"][pos]
+    return dict(force_max=float(fe.max()), force_rms=float(np.sqrt((fe ** 2).mean())),
+                force_raw_max_over_max=float(fr),
+                density_max=float(de.max()), density_rms=float(np.sqrt((de ** 2).mean())),
+                mindist_equal=bool(np.array_equal(mix["mindist"], ref["mindist"])),
+                v_max_abs_over_max=float(np.abs(mix["v"] - ref["v"]).max() / max(np.abs(ref["v"]).max(), 1e-300)))
+
+
+def interp_errors(tm, gen, ctx, col, field_seed=5):
+    """FP32 field + FP32 gradient interpolated, against FP64 field + FP64
+    gradient of the same (upcast) field"""
+    grid = col["grid"]
+    G = int(grid["G"])
+    f32 = tm.DeviceArray.from_host(ctx, gen.smooth_field(grid, field_seed, np.float32))
+    g32 = tm.DeviceArray(ctx, 3 * G ** 3, np.float32)
+    ctx.field_gradient(grid, f32, g32)
+    o32 = tm.DeviceArray(ctx, 4 * col["n"], np.float64)
+    ctx.cic_interp(grid, f32, g32, o32)
+    a = o32.get().reshape(-1, 4)
+    f64 = tm.DeviceArray.from_host(ctx, f32.get().astype(np.float64))
+    g64 = tm.DeviceArray(ctx, 3 * G ** 3, np.float64)
+    ctx.field_gradient(grid, f64, g64)
+    ctx.cic_interp(grid, f64, g64, o32)
+    b = o32.get().reshape(-1, 4)
+    for d in (f32, g32, o32, f64, g64):
+        d.free()
+    out = {}
+    for k, name in enumerate(("value", "gx", "gy", "gz")):
+        sc = max(np.abs(b[:, k]).max(), 1e-300)
+        e = np.abs(a[:, k] - b[:, k]) / sc
+        out[name] = dict(max=float(e.max()), rms=float(np.sqrt((e ** 2).mean())))
+    out["normalised_by"] = "max |FP64 value| per component"
+    return out
+
+
+def time_sph(tm, ctx, a, n, P):
     B = 96.0 * n + 4.0 * P
     out = {}
     for name, prec in (("mixed", tm.MIXED), ("fp64", tm.FP64)):
@@ -115,50 +198,21 @@ def main():
         ms = ctx.time_stage(tm.STAGE_SPH, prec, warmup=a.warmup, iters=a.steps)
         ctx.sync()
         wall = time.perf_counter() - t0
-        out[name] = dict(ms=ms, pairs_per_s=P / (ms * 1e-3), gbs=B / (ms * 1e-3) / 1e9, wall_s=wall)
-    # mixed-vs-FP64 error (FP64 GPU == CPU path bit for bit, see tests)
-    err = None
-    if "mixed" in out:
-        ctx.sph(tm.MIXED)
-        mix = ctx.results()
-        off, nbr = ctx.neighbor_lists()
-        rows = np.repeat(np.arange(n), np.diff(off))
-        X = np.stack([col["x"], col["y"], col["z"]], 1)
-        d = np.sqrt(((X[rows] - X[nbr]) ** 2).sum(1))
-        u = d / 2.0
-        w = np.where(u <= 0.5, -2 * u + 3 * u * u, -(1 - u) ** 2)
-        sabs = np.zeros(n)
-        np.add.at(sabs, rows, np.abs(w) * 48 / (np.pi * 16.0))
-        has = sabs > 0
-        fe = np.abs(mix["force"] - ref["force"]).max(1)[has] / sabs[has]
-        pos = ref["density"] > 0
-        de = np.abs(mix["density"] - ref["density"])[pos] / ref["density"][pos]
-        err = dict(force_max=float(fe.max()), force_rms=float(np.sqrt((fe ** 2).mean())),
-                   density_max=float(de.max()), density_rms=float(np.sqrt((de ** 2).mean())),
-                   v_max_abs_over_max=float(np.abs(mix["v"] - ref["v"]).max() / np.abs(ref["v"]).max()))
-        ctx.sph(tm.FP64)
-    # full step: sort + tree + neighbour lists + SPH (mixed) + CIC deposit 256^3 FP64
-    grid = col["grid"]
-    prod = tm.DeviceArray(ctx, int(grid["G"]) ** 3, np.float64)
-    ctx.cic_deposit(grid, prod, 0)
-    step_ms = ctx.time_stage(tm.STAGE_STEP, tm.MIXED, warmup=1, iters=3)
-    build_ms = ctx.time_stage(tm.STAGE_BUILD, tm.MIXED, warmup=1, iters=3)
-    nbr_ms = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
-    cic_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
-    full_ms = step_ms + cic_ms
+        out[name] = dict(ms=ms, pairs_per_s=P / (ms * 1e-3), gbs=B / (ms * 1e-3) / 1e9,
+                         roofline_frac=B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, wall_s=wall)
+    return out, B
 
-    cpu = None
-    if not a.no_cpu:
-        nthr, t_build, tp, tg = cpu_baseline(col, a.cpu_threads)
-        cpu = {"value": P / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
-               "sample": f"full workload ({n} cells, {P} pairs): computepotential + computegradient walks, "
-                         f"median of 3 after a warm-up; serial octbuild {t_build:.2f} s not included",
-               "t_potential_s": tp, "t_gradient_s": tg, "t_octbuild_s": t_build}
 
+def roofline(B, ms, traffic):
+    ach = B / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": traffic, "algorithmic_bytes_per_launch": B, "kernel": "k_sph (mixed)"}
+
+
+def line_common(a, out, n, P, maxnbr):
     mixed = out.get("mixed") or out.get("fp64")
-    traffic = traffic_from_profiles()
-    line = {
-        "metric": "SPH density+force pair-interactions/s, mixed vs FP64 plain, 1 MI355X",
+    return {
+        "metric": METRIC,
         "value": mixed["pairs_per_s"],
         "unit": "pair-interactions/s",
         "n_gpus": 1,
@@ -169,24 +223,245 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 pair evaluation, fp64 predicate and per-cell sums" if "mixed" in out else "f64",
-        "data": "synthetic (seeded Gaussian spheroid)",
-        "config": {"workload": "single colony, 2,000,000 cells, Gaussian spheroid W=146.6, r=1, h=2, mixed SPH"
-                   if a.scale == 1.0 else f"scaled {a.scale}", "cells": n, "pairs": P, "max_neighbours": maxnbr,
-                   "parallelism": "1 GPU"},
+        "data": "synthetic (seeded)",
         "fp64": out.get("fp64"),
         "mixed": out.get("mixed"),
         "speedup_mixed_vs_fp64": (out["fp64"]["ms"] / out["mixed"]["ms"]) if len(out) == 2 else None,
+    }
+
+
+def run_single(a, tm, gen):
+    col = gen.config(1, scale=a.scale)
+    n = col["n"]
+    ctx = tm.Context(0)
+    ctx.upload_colony(col)
+    ctx.step(tm.FP64)
+    P, maxnbr = ctx.pair_count()
+    ref = ctx.results()  # FP64 drop-in, bit-exact with the CPU path
+    tune = ctx.autotune(tm.MIXED) if a.autotune else None
+    log(f"single colony n={n} P={P} max_nbr={maxnbr}")
+    out, B = time_sph(tm, ctx, a, n, P)
+    err = None
+    if "mixed" in out:
+        ctx.sph(tm.MIXED)
+        mix = ctx.results()
+        off, nbr = ctx.neighbor_lists()
+        err = sph_errors(col, off, nbr, mix, ref)
+        ctx.sph(tm.FP64)
+    ierr = interp_errors(tm, gen, ctx, col)
+    # full step: sort + tree + neighbour lists + SPH (mixed) + CIC deposit 256^3 FP64
+    grid = col["grid"]
+    prod = tm.DeviceArray(ctx, int(grid["G"]) ** 3, np.float64)
+    ctx.cic_deposit(grid, prod, 0)
+    step_ms = ctx.time_stage(tm.STAGE_STEP, tm.MIXED, warmup=1, iters=3)
+    build_ms = ctx.time_stage(tm.STAGE_BUILD, tm.MIXED, warmup=1, iters=3)
+    nbr_ms = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
+    cic_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
+    prod.free()
+    copy_gbs = copy_bandwidth(tm, ctx)
+
+    cpu = None
+    if not a.no_cpu:
+        nthr, t_build, tp, tg = cpu_walks(col, a.cpu_threads)
+        cpu = {"value": P / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
+               "sample": f"full workload ({n} cells, {P} pairs): computepotential + computegradient walks, "
+                         f"median of 3 after a warm-up; serial octbuild {t_build:.2f} s not included",
+               "t_potential_s": tp, "t_gradient_s": tg, "t_octbuild_s": t_build}
+    line = line_common(a, out, n, P, maxnbr)
+    mixed = out.get("mixed") or out.get("fp64")
+    line.update({
+        "data": "synthetic (seeded Gaussian spheroid)",
+        "config": {"workload": "single colony, 2,000,000 cells, Gaussian spheroid W=146.6, r=1, h=2, mixed SPH"
+                   if a.scale == 1.0 else f"single colony scaled {a.scale}", "cells": n, "pairs": P,
+                   "max_neighbours": maxnbr, "parallelism": "1 GPU"},
         "mixed_vs_fp64_error": err,
-        "full_step_ms": {"total": full_ms, "build": build_ms, "neighbors": nbr_ms,
+        "interp_fp32_vs_fp64_error_256^3": ierr,
+        "full_step_ms": {"total": step_ms + cic_ms, "build": build_ms, "neighbors": nbr_ms,
                          "sph_mixed": out["mixed"]["ms"] if "mixed" in out else None, "cic_deposit_256^3": cic_ms},
-        "roofline": {"bound": "hbm", "achieved": B / (mixed["ms"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": B / (mixed["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": B},
+        "copy_kernel_gbs": copy_gbs,
+        "roofline": roofline(B, mixed["ms"], traffic_from_profiles()),
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (mixed["pairs_per_s"] / cpu["value"]) if cpu else None,
-    }
-    print(json.dumps(line), flush=True)
+        "autotune": tune,
+    })
+    line["roofline"]["frac_of_copy"] = line["roofline"]["achieved"] / copy_gbs
     ctx.close()
+    return line
+
+
+def run_batched(a, tm, gen):
+    cols = gen.colonies_config3(scale=a.scale)
+    ctx = tm.Context(0)
+    off_c = ctx.upload_colonies(cols)
+    n = int(off_c[-1])
+    ctx.step(tm.FP64)
+    P, maxnbr = ctx.pair_count()
+    ref = ctx.results()
+    log(f"batched: {len(cols)} colonies, n={n} P={P} max_nbr={maxnbr}")
+    tune = ctx.autotune(tm.MIXED) if a.autotune else None
+    out, B = time_sph(tm, ctx, a, n, P)
+    # graph-captured step: build + neighbour lists + mixed SPH for all colonies
+    ctx.graph_capture(tm.MIXED)
+    for _ in range(a.warmup):
+        ctx.graph_launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.graph_launch()
+    ctx.sync()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    eager_ms = ctx.time_stage(tm.STAGE_STEP, tm.MIXED, warmup=1, iters=max(3, a.steps // 4))
+    ctx.sph(tm.MIXED)
+    mix = ctx.results()
+    off, nbr = ctx.neighbor_lists()
+    # per-colony check against FP64
+    sabs = pair_abs_sum(dict(n=n, x=np.concatenate([c["x"] for c in cols]),
+                             y=np.concatenate([c["y"] for c in cols]), z=np.concatenate([c["z"] for c in cols]),
+                             ctype=np.concatenate([c["ctype"] for c in cols]), types=cols[0]["types"]), off, nbr)
+    worst, rms_acc, cnt = 0.0, 0.0, 0
+    dworst = 0.0
+    for i in range(len(cols)):
+        s, e = int(off_c[i]), int(off_c[i + 1])
+        has = sabs[s:e] > 0
+        fe = np.abs(mix["force"][s:e] - ref["force"][s:e]).max(1)[has] / sabs[s:e][has]
+        if fe.size:
+            worst = max(worst, float(fe.max()))
+            rms_acc += float((fe ** 2).sum())
+            cnt += fe.size
+        pos = ref["density"][s:e] > 0
+        de = np.abs(mix["density"][s:e] - ref["density"][s:e])[pos] / ref["density"][s:e][pos]
+        if de.size:
+            dworst = max(dworst, float(de.max()))
+    err = dict(force_max_worst_colony=worst, force_rms=float(np.sqrt(rms_acc / max(cnt, 1))),
+               density_max_worst_colony=dworst, mindist_equal=bool(np.array_equal(mix["mindist"], ref["mindist"])))
+    cpu = None
+    if not a.no_cpu:
+        # bounded sample: the first 32 colonies, one oracle octree and walk pair each
+        k = 32
+        tp = tg = tb = 0.0
+        Pk = int(off[int(off_c[k])])
+        nthr = 0
+        for c in cols[:k]:
+            nthr, t_build, p_, g_ = cpu_walks(c, a.cpu_threads)
+            tp, tg, tb = tp + p_, tg + g_, tb + t_build
+        cpu = {"value": Pk / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
+               "sample": f"first {k} of {len(cols)} colonies ({int(off_c[k])} cells, {Pk} pairs), "
+                         f"computepotential + computegradient walks per colony, median of 3",
+               "t_potential_s": tp, "t_gradient_s": tg, "t_octbuild_s": tb}
+    line = line_common(a, out, n, P, maxnbr)
+    mixed = out.get("mixed") or out.get("fp64")
+    line.update({
+        "data": "synthetic (seeded: uniform balls and Gaussian spheroids)",
+        "config": {"workload": f"batched: {len(cols)} colonies of 1,000..200,000 cells (log-uniform), "
+                               f"{n} cells, r=1, h=2, one launch per stage, graph-captured mixed step",
+                   "colonies": len(cols), "cells": n, "pairs": P, "max_neighbours": maxnbr, "parallelism": "1 GPU"},
+        "mixed_vs_fp64_error": err,
+        "graph_step_ms": graph_ms,
+        "eager_step_ms": eager_ms,
+        "roofline": roofline(B, mixed["ms"], None),
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (mixed["pairs_per_s"] / cpu["value"]) if cpu else None,
+        "autotune": tune,
+    })
+    ctx.close()
+    return line
+
+
+def time_host(ctx, fn, iters):
+    fn()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        fn()
+    ctx.sync()
+    return (time.perf_counter() - t0) * 1e3 / iters
+
+
+def run_large(a, tm, gen):
+    col = gen.config(4, scale=a.scale)
+    n = col["n"]
+    grid = col["grid"]
+    G = int(grid["G"])
+    log(f"large: n={n} grid {G}^3")
+    ctx = tm.Context(0)
+    ctx.upload_colony(col)
+    ctx.step(tm.FP64)
+    P, maxnbr = ctx.pair_count()
+    ref = ctx.results()
+    log(f"large: P={P} max_nbr={maxnbr}")
+    tune = ctx.autotune(tm.MIXED) if a.autotune else None
+    out, B = time_sph(tm, ctx, a, n, P)
+    err = None
+    if "mixed" in out:
+        ctx.sph(tm.MIXED)
+        mix = ctx.results()
+        off, nbr = ctx.neighbor_lists()
+        err = sph_errors(col, off, nbr, mix, ref)
+        del off, nbr
+    build_ms = ctx.time_stage(tm.STAGE_BUILD, tm.MIXED, warmup=1, iters=3)
+    nbr_ms = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
+    prod = tm.DeviceArray(ctx, G ** 3, np.float64)
+    ctx.cic_deposit(grid, prod, 1)
+    dep_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
+    mass = float(prod.get().sum())
+    vol = float((4.0 / 3.0 * np.pi * col["size"] ** 3).sum())
+    prod.free()
+    f32 = tm.DeviceArray.from_host(ctx, gen.smooth_field(grid, 5, np.float32))
+    g32 = tm.DeviceArray(ctx, 3 * G ** 3, np.float32)
+    o = tm.DeviceArray(ctx, 4 * n, np.float64)
+    grad_ms = time_host(ctx, lambda: ctx.field_gradient(grid, f32, g32), 5)
+    interp_ms = time_host(ctx, lambda: ctx.cic_interp(grid, f32, g32, o), 5)
+    for d in (f32, g32, o):
+        d.free()
+    ierr = interp_errors(tm, gen, ctx, col)
+    cpu = None
+    if not a.no_cpu:
+        smp = gen.config(4, scale=0.04 * a.scale)
+        Ps = gpu_pairs(tm, smp)
+        nthr, t_build, tp, tg = cpu_walks(smp, a.cpu_threads)
+        cpu = {"value": Ps / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
+               "sample": f"same generator at 4% scale ({smp['n']} cells, {Ps} pairs, same density and type mix): "
+                         f"computepotential + computegradient walks, median of 3",
+               "t_potential_s": tp, "t_gradient_s": tg, "t_octbuild_s": t_build}
+    line = line_common(a, out, n, P, maxnbr)
+    mixed = out.get("mixed") or out.get("fp64")
+    G3 = G ** 3
+    line.update({
+        "data": "synthetic (seeded Gaussian spheroid, 5% type 1)",
+        "config": {"workload": f"large single colony: {n} cells, 5% radius 1.5 (h 3), Gaussian W=428.7, "
+                               f"mixed SPH + CIC volume deposit {G}^3 FP64 + FP32 field gradient/interpolation {G}^3",
+                   "cells": n, "pairs": P, "max_neighbours": maxnbr, "grid": G, "parallelism": "1 GPU"},
+        "mixed_vs_fp64_error": err,
+        "interp_fp32_vs_fp64_error": ierr,
+        "stage_ms": {"build": build_ms, "neighbors": nbr_ms, "sph_mixed": out.get("mixed", {}).get("ms"),
+                     "sph_fp64": out.get("fp64", {}).get("ms"), "cic_deposit_volume": dep_ms,
+                     "field_gradient_f32": grad_ms, "cic_interp_f32": interp_ms},
+        # algorithmic bytes: deposit 32 N + 8 G, gradient (4 + 12) G, interpolation 56 N + 16 G
+        "stage_gbs": {"cic_deposit_volume": (32.0 * n + 8.0 * G3) / (dep_ms * 1e-3) / 1e9,
+                      "field_gradient_f32": 16.0 * G3 / (grad_ms * 1e-3) / 1e9,
+                      "cic_interp_f32": (56.0 * n + 16.0 * G3) / (interp_ms * 1e-3) / 1e9},
+        "deposit_mass_rel_err": abs(mass - vol) / vol,
+        "roofline": roofline(B, mixed["ms"], None),
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (mixed["pairs_per_s"] / cpu["value"]) if cpu else None,
+        "autotune": tune,
+    })
+    ctx.close()
+    return line
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import timothy_amd as tm
+    from timothy_amd import generators as gen
+
+    run = {"single": run_single, "batched": run_batched, "large": run_large}[a.workload]
+    line = run(a, tm, gen)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -67,8 +67,25 @@ def build(verbose: bool = False) -> None:
           os.path.join(CSRC, "compat", "tm_compat.c"), "-o", tmp,
           "-L" + HERE, "-ltimothy_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
     os.replace(tmp, COMPAT)
+    build_step_harness()
     if verbose:
-        print("built", LIB, COMPAT)
+        print("built", LIB, COMPAT, HARNESS)
+
+
+HARNESS = os.path.join(ROOT, "tests", "compat", "_build", "step_harness")
+
+
+def build_step_harness() -> None:
+    """The C program the drop-in tests run: it links libtimothy_compat.so the
+    way the reference's own executable would link octree.o/potential.o/..."""
+    src = os.path.join(ROOT, "tests", "compat", "step_harness.c")
+    if not os.path.exists(src):
+        return
+    os.makedirs(os.path.dirname(HARNESS), exist_ok=True)
+    tmp = HARNESS + ".tmp"
+    _run(["gcc", "-std=gnu11", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp,
+          "-L" + HERE, "-ltimothy_compat", "-ltimothy_hip", "-Wl,-rpath,$ORIGIN/../../../timothy_amd", "-lm"])
+    os.replace(tmp, HARNESS)
 
 
 if __name__ == "__main__":
