@@ -102,9 +102,11 @@ tm_status tm_sync(tm_ctx *ctx);
 /* dimension 2 or 3 (src/global.h:177); ntypes <= 16 */
 tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes, int32_t dimension);
 /* tunables: "nbr_cap" neighbour slots per cell (grows on overflow),
- * "sph_block" / "nbr_block" workgroup sizes, "sph_lanes" lanes per cell of the
- * mixed kernel (1 = one lane per cell), "mixed_recentre" (1 = FP32 positions
- * re-centred per 64-cell block, 0 = FP64 positions) */
+ * "sph_block" / "nbr_block" workgroup sizes (64, 128, 256), "sph_unroll_fp64" /
+ * "sph_unroll_mixed" neighbours fetched per trip of the pair kernels (1, 2, 4,
+ * 8), "sph_lanes" lanes per cell of the mixed kernel (1 = one lane per cell),
+ * "sph_balance" (1 = the FP64 kernel ranks the cells of a workgroup by list
+ * length); read-only "nbr_alloc" (slots per cell of the current lists) */
 tm_status tm_set_param(tm_ctx *ctx, const char *key, int64_t value);
 tm_status tm_get_param(tm_ctx *ctx, const char *key, int64_t *value);
 

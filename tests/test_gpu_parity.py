@@ -89,6 +89,52 @@ def test_build_and_neighbors_exact(tm, oracle_mod, ctx, name, make):
     say(f"  pairs={P} candidates={len(cand)} max_nbr={mx}")
 
 
+def test_launch_shapes_invariant(tm, oracle_mod, ctx):
+    """Walk workgroup size, pair-kernel workgroup size, neighbours per trip,
+    lanes per cell and workgroup ranking change nothing: lists, qstar and
+    mindist stay equal to the oracle's, FP64 stays bit-exact, and the mixed
+    sums stay within their bound."""
+    O = oracle_mod
+    col = CASES[1][1]()
+    ctx.upload_colony(col)
+    ctx.build()
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    ref = oracle_results(tree)
+    coff, cand, inh = tree.candidates()
+    exp_q = np.array([cand[coff[p + 1] - 1] if coff[p + 1] > coff[p] else -1 for p in range(col["n"])])
+    ctx.neighbors()
+    lists = ctx.neighbor_lists()
+    for nb in (64, 128, 256):
+        say(f"nbr_block {nb}")
+        ctx.set_param("nbr_block", nb)
+        ctx.neighbors()
+        got = ctx.neighbor_lists()
+        assert np.array_equal(got[0], lists[0]) and np.array_equal(got[1], lists[1])
+        assert np.array_equal(ctx.qstar(), exp_q)
+    for bs in (64, 256, 128):
+        for u in (1, 2, 8, 4):
+            for bal in (0, 1):
+                say(f"fp64 block {bs} unroll {u} balance {bal}")
+                ctx.set_param("sph_block", bs)
+                ctx.set_param("sph_unroll_fp64", u)
+                ctx.set_param("sph_balance", bal)
+                ctx.sph(tm.FP64)
+                r = ctx.results()
+                for k in ("density", "v", "mindist", "force"):
+                    assert np.array_equal(r[k], ref[k]), k
+    for lanes in (2, 4, 8, 1):
+        for u in (1, 8, 4):
+            say(f"mixed lanes {lanes} unroll {u}")
+            ctx.set_param("sph_lanes", lanes)
+            ctx.set_param("sph_unroll_mixed", u)
+            ctx.sph(tm.MIXED)
+            r = ctx.results()
+            assert np.array_equal(r["mindist"], ref["mindist"])
+            assert np.abs(r["force"] - ref["force"]).max() <= 1e-5 * np.abs(ref["force"]).max()
+            assert np.abs(r["density"] - ref["density"]).max() <= 1e-5 * np.abs(ref["density"]).max()
+    ctx.set_param("sph_unroll_mixed", 8)
+
+
 @pytest.mark.parametrize("name,make", CASES)
 def test_sph_fp64_bit_exact(tm, oracle_mod, ctx, name, make):
     col = make()

@@ -36,8 +36,6 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
   free_ptr(c->d_perm);
   free_ptr(c->d_code);
   free_ptr(c->d_pos);
-  free_ptr(c->d_posf);
-  free_ptr(c->d_origin);
   free_ptr(c->d_young_s);
   free_ptr(c->d_ctype_s);
   free_ptr(c->d_phase_s);
@@ -68,7 +66,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
   } list[] = {
       {(void **)&c->d_key, 8},      {(void **)&c->d_key_alt, 8},  {(void **)&c->d_val_alt, 4},
       {(void **)&c->d_perm, 4},     {(void **)&c->d_code, 16},    {(void **)&c->d_pos, 32},
-      {(void **)&c->d_posf, 16},    {(void **)&c->d_young_s, 4},  {(void **)&c->d_ctype_s, 4},
+      {(void **)&c->d_young_s, 4},  {(void **)&c->d_ctype_s, 4},
       {(void **)&c->d_phase_s, 4},  {(void **)&c->d_span, 4},     {(void **)&c->d_nodes, 64},
       {(void **)&c->d_range, 8},    {(void **)&c->d_delta, 4},    {(void **)&c->d_ncount, 4},
       {(void **)&c->d_qstar, 4},    {(void **)&c->d_dens, 8},     {(void **)&c->d_v, 8},
@@ -81,10 +79,6 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
       *a.p = nullptr;
       return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed for per-cell arrays");
     }
-  }
-  if (hipMalloc((void **)&c->d_origin, sizeof(double4) * (size_t)(cap / 64 + 2)) != hipSuccess) {
-    c->d_origin = nullptr;
-    return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed for block origins");
   }
   c->cap_n = cap;
   return TM_OK;
@@ -293,8 +287,6 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_perm);
   free_ptr(c->d_code);
   free_ptr(c->d_pos);
-  free_ptr(c->d_posf);
-  free_ptr(c->d_origin);
   free_ptr(c->d_young_s);
   free_ptr(c->d_ctype_s);
   free_ptr(c->d_phase_s);
@@ -333,6 +325,10 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_aux_o);
   free_ptr(c->d_chunk_start);
   free_ptr(c->d_chunk_cid);
+  free_ptr(c->d_tile_idx);
+  free_ptr(c->d_tile_cnt);
+  free_ptr(c->d_tile_cell);
+  free_ptr(c->d_nbr16);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -378,10 +374,9 @@ tm_status tm_set_celltypes(tm_ctx *c, const tm_celltype *types, int32_t ntypes, 
       d.cF = (40 * 8) / (7 * M_PI * t.h3);
     }
     d.cut2 = cutoff_sq(t.h);
+    d.hinv = 1.0 / t.h;  // correctly rounded reciprocals: the divisor of u = d/h and of r_q/size
+    d.sinv = 1.0 / t.size;
     d.hinv_f = (float)(1.0 / t.h);
-    d.cW_f = (float)d.cW;
-    d.cF_f = (float)d.cF;
-    d.sc_over_size_f = (float)(d.sc / t.size);
   }
   c->ntypes = ntypes;
   c->dim = dimension;
@@ -410,8 +405,8 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->sph_lanes = (int)v;
   } else if (!strcmp(key, "sph_balance")) {
     c->sph_balance = v ? 1 : 0;
-  } else if (!strcmp(key, "mixed_recentre")) {
-    c->mixed_recentre = v ? 1 : 0;
+  } else if (!strcmp(key, "sph_tiles")) {
+    c->sph_tiles = v ? 1 : 0;
   } else {
     return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
   }
@@ -427,7 +422,8 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "nbr_block")) *v = c->nbr_block;
   else if (!strcmp(key, "sph_lanes")) *v = c->sph_lanes;
   else if (!strcmp(key, "sph_balance")) *v = c->sph_balance;
-  else if (!strcmp(key, "mixed_recentre")) *v = c->mixed_recentre;
+  else if (!strcmp(key, "sph_tiles")) *v = c->sph_tiles;
+  else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
   else return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
   return TM_OK;

@@ -187,6 +187,7 @@ __global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm,
                                                double4 *__restrict__ pos, float *__restrict__ young_s,
                                                int32_t *__restrict__ ctype_s, int32_t *__restrict__ phase_s) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i == 0) pos[n] = make_double4(TM_FAR, TM_FAR, TM_FAR, 0.0);  // stand-in neighbour padding the lists
   if (i >= n) return;
   int64_t o = perm[i];
   if (o < 0 || o >= n) o = 0;  // unreachable for a valid permutation; bounds the gathers

@@ -17,6 +17,18 @@
 #define TM_FLAG_CTYPE 4
 #define TM_FLAG_WALK 8
 #define TM_BBOX_CHUNK 8192
+// LDS tiles: the pair kernels take cells in tiles of TM_TILE consecutive
+// (Morton-order) cells; the distinct neighbours of a tile, at most
+// TM_TILE_MAX, are staged once in LDS and the lists hold 16-bit tile-local
+// indices.  A tile with more distinct neighbours is marked TM_TILE_OVER and
+// read through the 32-bit lists.
+#define TM_TILE 256
+#define TM_TILE_MAX 1536
+#define TM_TILE_OVER 0xFFFFFFFFu
+// coordinate of the stand-in record pos[n] that pads neighbour lists: far
+// enough that a pair with it lies beyond every cutoff, near enough that its
+// squared distance stays finite in FP32
+#define TM_FAR 1.0e12
 
 // Per cell-type constants, derived on the host with the reference's own
 // expressions (src/inline.h:45-46,71-72) so the device reads identical values.
@@ -26,7 +38,8 @@ struct TypeDev {
   double cW;       // kernel normalisation 8/(pi h3) (3D) or 40/(7 pi h2) (2D)
   double cF;       // gradient normalisation 48/(pi h4) (3D) or 320/(7 pi h3) (2D)
   double cut2;     // largest d2 with RN(sqrt(d2)) <= h: exact form of dist <= h
-  float hinv_f, cW_f, cF_f, sc_over_size_f;
+  double hinv, sinv;  // RN(1/h), RN(1/size)
+  float hinv_f;
 };
 
 // 32-byte tree node: the region [lo, hi) in location-code units per axis,
@@ -58,7 +71,6 @@ struct tm_ctx {
   int nbr_block = 256;
   int sph_lanes = 1;
   int sph_balance = 1;  // FP64 kernel: rank cells by list length inside each workgroup
-  int mixed_recentre = 0;
 
   // cells (reference order)
   int64_t n = 0, cap_n = 0;
@@ -97,8 +109,6 @@ struct tm_ctx {
   int32_t *d_perm = nullptr;
   uint4 *d_code = nullptr;  // x, y, z codes, colony id
   double4 *d_pos = nullptr; // x, y, z, size
-  float4 *d_posf = nullptr; // block-recentred FP32 positions, size
-  double4 *d_origin = nullptr; // per 64-cell block origin (xyz) and extent flag (w)
   float *d_young_s = nullptr;
   int32_t *d_ctype_s = nullptr, *d_phase_s = nullptr;
   int32_t *d_span = nullptr;
@@ -111,12 +121,21 @@ struct tm_ctx {
   int32_t *d_parent = nullptr;  // parent of every node (2n-1), -1 at roots
   int64_t cap_tree_aux = 0;
 
-  // neighbours (slot-major: nbr[k * n + p])
+  // neighbours: slot k of cell p at nbr[tm_slot(p, k, nbr_alloc)]
   uint32_t *d_nbr = nullptr;
   int64_t nbr_alloc = 0;  // slots per cell of the current layout
   int64_t nbr_elems = 0;  // elements allocated (>= n * nbr_alloc)
   uint32_t *d_ncount = nullptr;
   int32_t *d_qstar = nullptr;
+
+  // LDS tiles built from the lists (see TM_TILE)
+  int sph_tiles = 1;
+  uint32_t *d_tile_idx = nullptr;   // per tile: distinct neighbours, ascending (TM_TILE_MAX slots)
+  uint32_t *d_tile_cnt = nullptr;   // per tile: their number, or TM_TILE_OVER
+  uint8_t *d_tile_cell = nullptr;   // per tile: cell offset of each rank (longest list first)
+  uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_slot(g*TM_TILE + r, k, nbr_alloc)
+  int64_t cap_tile_idx = 0, cap_tile_cnt = 0, cap_tile_cell = 0, cap_nbr16 = 0;
+  uint32_t tile_lds = 0;            // neighbour records the pair kernel stages per tile (0: no tiles built)
 
   // outputs, Morton order
   double *d_dens = nullptr, *d_v = nullptr, *d_mind = nullptr;
@@ -180,6 +199,7 @@ tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n);
 tm_status tm_scan_u32(tm_ctx *c, const uint32_t *in, uint32_t *out, int64_t n);
 tm_status tm_run_build(tm_ctx *c);
 tm_status tm_run_neighbors(tm_ctx *c, bool check);
+tm_status tm_run_tiles(tm_ctx *c, bool check);
 tm_status tm_run_sph(tm_ctx *c, tm_precision prec);
 tm_status tm_run_unpermute(tm_ctx *c);
 tm_status tm_run_deposit(tm_ctx *c, const tm_grid *g, int weight_mode, double *prod);

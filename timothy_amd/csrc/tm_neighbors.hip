@@ -5,8 +5,9 @@
 //
 // Per cell p this yields:
 //   N(p)    candidates q != p with sqrt((dx*dx+dy*dy)+dz*dz) <= h_p, stored in
-//           visit order in slot-major lists nbr[k*n + p] (coalesced for the
-//           one-lane-per-cell pair kernels), count[p]
+//           visit order at nbr[tm_slot(p, k, cap)] (slot-major inside each
+//           group of 64 cells, coalesced for the one-lane-per-cell pair
+//           kernels), count[p]
 //   mindist min distance over all candidates (src/potential.c:49), computed as
 //           sqrt of the minimum squared distance (sqrt is monotone and
 //           correctly rounded, so this is the minimum of the rounded roots)
@@ -29,11 +30,12 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
                                                   const int32_t *__restrict__ colroot,
                                                   const double4 *__restrict__ aff,
                                                   const TypeDev *__restrict__ types, int64_t n, int64_t cap,
-                                                  uint32_t *__restrict__ nbr, uint32_t *__restrict__ count,
-                                                  double *__restrict__ mindist, int32_t *__restrict__ qstar,
-                                                  unsigned int *__restrict__ flags) {
-  int64_t p = tm_xcd_block() * BS + threadIdx.x;
-  if (p >= n) return;
+                                                  uint32_t *__restrict__ nbr,
+                                                  uint32_t *__restrict__ count, double *__restrict__ mindist,
+                                                  int32_t *__restrict__ qstar, unsigned int *__restrict__ flags) {
+  const int64_t cell = tm_xcd_block() * BS + threadIdx.x;
+  const bool live = cell < n;
+  const int64_t p = live ? cell : n - 1;  // idle lanes only join the padding step below
   const double4 P = pos[p];
   const uint32_t cid = code[p].w;
   const double4 A = aff[cid];
@@ -46,13 +48,27 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   const uint32_t mn2 = box_code(P.z - h, A.z, A.w), mx2 = box_code(P.z + h, A.z, A.w);
   const uint32_t leaf0 = (uint32_t)(n - 1);
   const uint32_t nnodes = (uint32_t)(2 * n - 1);
-  uint32_t node = (uint32_t)colroot[cid];
+  uint32_t node = live ? (uint32_t)colroot[cid] : TM_END;
   uint32_t cnt = 0;
   int32_t qs = -1;
   double md2 = 1.0 / 0.0;
   // a well-formed walk visits each node at most once; the budget only stops a
   // malformed tree from looping
   int64_t budget = 2 * n + 8;
+  // candidate q (its leaf octant overlaps the box): distance, mindist, the
+  // last-visited candidate, and the list entry when within the cutoff
+  auto visit = [&](uint32_t q) {
+    if (q == (uint32_t)p) return;
+    const double4 Q = pos[q];
+    const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+    const double d2 = (dx * dx + dy * dy) + dz * dz;
+    md2 = d2 < md2 ? d2 : md2;
+    qs = (int32_t)q;
+    if (d2 <= cut2) {
+      if (cnt < cap) nbr[tm_slot(p, cnt, cap)] = q;
+      ++cnt;
+    }
+  };
   while (node != TM_END) {
     if (node >= nnodes || --budget < 0) {
       atomicOr(flags, (unsigned)TM_FLAG_WALK);
@@ -65,22 +81,24 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
       node = a.w;
       continue;
     }
-    if (hit) {
-      const uint32_t q = node - leaf0;
-      if (q != (uint32_t)p) {
-        const double4 Q = pos[q];
-        const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-        const double d2 = (dx * dx + dy * dy) + dz * dz;
-        md2 = d2 < md2 ? d2 : md2;
-        qs = (int32_t)q;
-        if (d2 <= cut2) {
-          if (cnt < cap) nbr[tm_slot(p, cnt, cap)] = q;
-          ++cnt;
-        }
-      }
-    }
+    if (hit) visit(node - leaf0);
     node = b.w;
   }
+  // Pad each list to the longest one of its 64-cell slot group (this wave)
+  // with cell index n, the far-away stand-in record the build places after
+  // the cells: the pair kernels then read whole slot rows without per-lane
+  // tests, and a stand-in pair contributes exactly zero to the mixed sums.
+  // (rows are written wave-wide, one 256-byte row per step)
+  const uint32_t used = min(cnt, (uint32_t)cap);
+  uint32_t g = used, lo = live ? used : 0xFFFFFFFFu;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    g = max(g, (uint32_t)__shfl_xor((int)g, o, 64));
+    lo = min(lo, (uint32_t)__shfl_xor((int)lo, o, 64));
+  }
+  for (uint32_t k = lo; k < g; ++k)
+    if (live && k >= used) nbr[tm_slot(p, k, cap)] = (uint32_t)n;
+  if (!live) return;
   count[p] = cnt;
   mindist[p] = qs >= 0 ? sqrt(md2) : 1.7976931348623157e308;
   qstar[p] = qs;
@@ -114,18 +132,18 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     if (bs == 64)
       k_neighbors<64><<<tm_blocks(n, 64), 64, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
                                                                c->d_colroot, c->d_aff, c->d_types, n, c->nbr_alloc,
-                                                               c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar,
-                                                               c->d_flags);
+                                                               c->d_nbr, c->d_ncount,
+                                                               c->d_mind, c->d_qstar, c->d_flags);
     else if (bs == 128)
       k_neighbors<128><<<tm_blocks(n, 128), 128, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
                                                                   c->d_colroot, c->d_aff, c->d_types, n,
-                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind,
-                                                                  c->d_qstar, c->d_flags);
+                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount,
+                                                                  c->d_mind, c->d_qstar, c->d_flags);
     else
       k_neighbors<256><<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
                                                                   c->d_colroot, c->d_aff, c->d_types, n,
-                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind,
-                                                                  c->d_qstar, c->d_flags);
+                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount,
+                                                                  c->d_mind, c->d_qstar, c->d_flags);
     TM_HIP(c, hipGetLastError());
     if (!check) break;
     TM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
@@ -140,6 +158,7 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     if (mx <= c->nbr_alloc) break;
     c->nbr_cap = (mx + 7) & ~7ll;  // grow to the largest count seen, rounded up
   }
+  TM_TRY(tm_run_tiles(c, check));
   c->stage = 3;
   return TM_OK;
 }

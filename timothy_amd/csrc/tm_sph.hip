@@ -61,72 +61,232 @@ __device__ __forceinline__ float pot_fp32(double4 P, float yp, double4 Q, float 
   return (0.4f * xc * xc) * __builtin_amdgcn_rcpf(D) * (red * __builtin_amdgcn_rsqf(red)) + 0.1f * lens;
 }
 
+// a / b correctly rounded from y = RN(1/b): q0 = RN(a y) is within one ulp of
+// a / b, the remainder a - b q0 is exact in one FMA, and RN(q0 + r y) is the
+// correctly rounded quotient (Markstein, IBM J. Res. Dev. 34(1), 1990,
+// Theorem 1), i.e. the same double as the IEEE division a / b.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double r = __builtin_fma(-q0, b, a);
+  return __builtin_fma(r, y, q0);
+}
+
 // one pair of the FP64 drop-in: sphkernel (src/inline.h:40-51) into the
 // density and sphgradient (src/inline.h:59-83) into the force, expressions
-// and evaluation order as in the reference; 0 <= u <= 1 for listed neighbours
-__device__ __forceinline__ void pair_fp64(const double4 &P, const double4 &Q, double h, double sc, double csize,
-                                          double cW, double cF, bool dim3, double &de, double &f0, double &f1,
-                                          double &f2) {
+// and evaluation order as in the reference; 0 <= u <= 1 for listed
+// neighbours.  The five divisions of the reference (d/h, r_q/size and the
+// three force components by d) give the same doubles through div_rn: 1/h and
+// 1/size are per-type constants, 1/d is one IEEE division shared by the
+// three components.
+struct TypeFP64 {
+  double h, hinv, sc, size, sinv, cW, cF;
+};
+
+__device__ __forceinline__ void pair_fp64(const double4 &P, const double4 &Q, const TypeFP64 &T, bool dim3,
+                                          double &de, double &f0, double &f1, double &f2) {
   const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
   const double dist = sqrt((dx * dx + dy * dy) + dz * dz);
-  const double u = dist / h;
+  const double u = div_rn(dist, T.h, T.hinv);
+  const double cW = T.cW, cF = T.cF;
   double W;
   if (u <= 0.5) W = cW * (1 - 6 * u * u + 6 * u * u * u);
   else W = cW * (2 * (1 - u) * (1 - u) * (1 - u));
-  de += sc * (Q.w / csize) * (double)(float)W;
+  de += T.sc * div_rn(Q.w, T.size, T.sinv) * (double)(float)W;
   double w;
   if (u <= 0.5) w = -2.0 * u + 3.0 * u * u;
   else w = -(1.0 - u) * (1.0 - u);
-  f0 += w * cF * (Q.x - P.x) / dist;
-  f1 += w * cF * (Q.y - P.y) / dist;
-  if (dim3) f2 += w * cF * (Q.z - P.z) / dist;
+  const double rd = 1.0 / dist;
+  f0 += div_rn(w * cF * (Q.x - P.x), dist, rd);
+  f1 += div_rn(w * cF * (Q.y - P.y), dist, rd);
+  if (dim3) f2 += div_rn(w * cF * (Q.z - P.z), dist, rd);
 }
 
-// one pair of the mixed form (FP32 evaluation, FP64 separation and sums)
-__device__ __forceinline__ void pair_mixed(const double4 &P, const double4 &Q, float hinv, float hf, float hhalf,
-                                           double hh, float cW, float cF, float scs, bool dim3, double &de,
-                                           double &f0, double &f1, double &f2) {
-  // separation and squared distance in FP64, rounded once to FP32
+// One pair of the mixed form without its per-type constants, which are
+// applied once per cell: the density weight size_q * W(u) and the force
+// coefficient w(u) / d in FP32, with the FP32 separation.  The separation and
+// d^2 are formed in FP64 and rounded once.  With om = 1 - u and
+// m = max(1 - 2u, 0), the two pieces of the cubic spline combine into
+//   W = 2 om^3 - m^3   and   w = -(dW/du)/6 = m^2 - om^2,
+// exact for both u <= 1/2 (where 2 om^3 - (1-2u)^3 = 1 - 6u^2 + 6u^3) and
+// u > 1/2 (m = 0), so no lane of a wave branches.  om comes from the FP64
+// difference of squares, om = (h^2 - d^2) / ((h + d) h), so a near-cutoff
+// term keeps its relative accuracy.  Beyond the cutoff both vanish exactly
+// (om = m = 0), which is what makes the far stand-in record a zero term.
+struct MixedTerm {
+  float dw, coef, dx, dy, dz;
+};
+
+__device__ __forceinline__ MixedTerm term_mixed(const double4 &P, const double4 &Q, float m2hinv, float hf,
+                                                float hhf, double hh) {
   const double ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
   const double d2d = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
-  const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
   const float d2 = (float)d2d;
   const float rinv = __builtin_amdgcn_rsqf(d2);
   const float d = d2 * rinv;
-  float W, w;
-  if (d <= hhalf) {
-    const float u = d * hinv;
-    W = __builtin_fmaf(u * u, __builtin_fmaf(6.0f, u, -6.0f), 1.0f);
-    w = u * __builtin_fmaf(3.0f, u, -2.0f);
-  } else {
-    // 1 - u = (h^2 - d^2) / ((h + d) h): the difference of squares in FP64
-    // keeps the outer branch accurate up to the cutoff
-    const float om = fmaxf((float)(hh - d2d), 0.0f) * __builtin_amdgcn_rcpf((hf + d) * hf);
-    W = 2.0f * om * om * om;
-    w = -om * om;
-  }
-  de += (double)((scs * (float)Q.w) * (cW * W));
-  const float coef = w * cF * rinv;
-  f0 += (double)(coef * dx);
-  f1 += (double)(coef * dy);
-  if (dim3) f2 += (double)(coef * dz);
+  const float om = fmaxf((float)(hh - d2d), 0.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(d, hf, hhf));
+  const float m = fmaxf(__builtin_fmaf(d, m2hinv, 1.0f), 0.0f);
+  const float om2 = om * om, m2 = m * m;
+  MixedTerm t;
+  t.dw = (float)Q.w * __builtin_fmaf(om2 + om2, om, -(m2 * m));
+  t.coef = (m2 - om2) * rinv;
+  t.dx = (float)ex;
+  t.dy = (float)ey;
+  t.dz = (float)ez;
+  return t;
 }
 
-// Pair kernels.  L consecutive lanes share one cell and take its list slots
-// sub, sub+L, sub+2L, ... (L = 1 for the FP64 drop-in, which must add the
-// pairs in list order).  Each trip issues U position gathers for slots whose
-// indices were loaded one trip earlier, then loads the next trip's indices,
-// then evaluates the U pairs in order: the index round trip to memory overlaps
-// the gathers and the arithmetic instead of preceding them.  With L > 1 the
-// lanes' FP64 partial sums are added across the group at the end.
-template <bool MIXED, int U, int L>
+typedef unsigned int tm_u32x4 __attribute__((ext_vector_type(4)));
+typedef double tm_f64x2 __attribute__((ext_vector_type(2)));
+
+// Position record q.  With BUF the array is addressed through a buffer
+// resource and a 32-bit byte offset (one shift per gather instead of 64-bit
+// address arithmetic; reads past the resource's end return zeros); this needs
+// 32 n < 2^32.
+template <bool BUF>
+__device__ __forceinline__ double4 gather(const double4 *__restrict__ pos, __amdgpu_buffer_rsrc_t rs, uint32_t q) {
+  if constexpr (BUF) {
+    const uint32_t o = q << 5;
+    const tm_f64x2 a = __builtin_bit_cast(tm_f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
+    const tm_f64x2 b = __builtin_bit_cast(tm_f64x2, __builtin_amdgcn_raw_buffer_load_b128(rs, o + 16u, 0, 0));
+    return make_double4(a.x, a.y, b.x, b.y);
+  } else {
+    return pos[q];
+  }
+}
+
+// Per-cell sums over one cell's list, shared by both pair kernels.  L
+// consecutive lanes share one cell and take its list slots sub, sub+L, ...
+// (L = 1 for the FP64 drop-in, which must add the pairs in list order).  The
+// trips of a wave run to the longest list among its cells (wmax, wave-
+// uniform).  slot(kb) returns the record number in slot kb + sub (kb wave-
+// uniform), fetch(q) the record.  Each trip fetches U records whose numbers
+// were read one trip earlier, then reads the next trip's numbers, then
+// evaluates the U pairs in list order.  The mixed form adds a trip's U terms
+// in FP32 and the trip's sums into the FP64 per-cell accumulators; its
+// padding slots hold the far stand-in record, whose terms are exactly zero.
+// The FP64 form evaluates only the slots below the cell's own count.
+template <bool MIXED, int U, int L, bool DIM3, class Slot, class Fetch>
+__device__ __forceinline__ void cell_sums(const double4 &P, const TypeFP64 &C64, float hf, float m2hinv, float hhf,
+                                          double hh, uint32_t cnt, uint32_t wmax, uint32_t far, Slot slot,
+                                          Fetch fetch, double &de, double &f0, double &f1, double &f2) {
+  constexpr uint32_t STEP = (uint32_t)(U * L);
+  uint32_t nxt[U];
+#pragma unroll
+  for (int j = 0; j < U; ++j) nxt[j] = (uint32_t)(L * j) < wmax ? slot((uint32_t)(L * j)) : far;
+  for (uint32_t k0 = 0; k0 < wmax; k0 += STEP) {
+    double4 Q[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) Q[j] = fetch(nxt[j]);
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const uint32_t kb = k0 + STEP + (uint32_t)(L * j);
+      nxt[j] = kb < wmax ? slot(kb) : far;
+    }
+    if constexpr (MIXED) {
+      float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const MixedTerm m = term_mixed(P, Q[j], m2hinv, hf, hhf, hh);
+        sw += m.dw;
+        s0 = __builtin_fmaf(m.coef, m.dx, s0);
+        s1 = __builtin_fmaf(m.coef, m.dy, s1);
+        if (DIM3) s2 = __builtin_fmaf(m.coef, m.dz, s2);
+      }
+      de += (double)sw;
+      f0 += (double)s0;
+      f1 += (double)s1;
+      if (DIM3) f2 += (double)s2;
+    } else {
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (k0 + (uint32_t)j < cnt) pair_fp64(P, Q[j], C64, DIM3, de, f0, f1, f2);
+    }
+  }
+}
+
+// wave-wide maximum (every lane of the wave takes part)
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// The cell's outputs: density, v (the potential with its first-visited
+// candidate qstar, src/potential.c:124) and force.
+template <bool MIXED>
+__device__ __forceinline__ void store_cell(int64_t p, const double4 &P, const TypeDev &T, double de, double f0,
+                                           double f1, double f2, const double4 *__restrict__ pos,
+                                           const float *__restrict__ young_s, const int32_t *__restrict__ qstar,
+                                           double *__restrict__ dens, double *__restrict__ vout,
+                                           double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
+  const int32_t qs = qstar[p];
+  if (MIXED) {
+    dens[p] = de * (T.sc / T.size * T.cW);
+    vout[p] = qs >= 0 ? (double)pot_fp32(P, young_s[p], pos[qs], young_s[qs], T.cut2) : 0.0;
+    fx[p] = f0 * T.cF;
+    fy[p] = f1 * T.cF;
+    fz[p] = f2 * T.cF;
+  } else {
+    dens[p] = de;
+    vout[p] = qs >= 0 ? pot_fp64(P, young_s[p], pos[qs], young_s[qs], T.h) : 0.0;
+    fx[p] = f0;
+    fy[p] = f1;
+    fz[p] = f2;
+  }
+}
+
+// Sums of the cells whose 32-bit lists live in the slot groups from g0 on,
+// records gathered from the position array: the pair kernel for tiles that
+// are not staged, and for launches without tiles.  lane_off is this lane's
+// byte offset in the list descriptor rl.
+template <bool MIXED, int U, int L, bool DIM3, bool BUF>
+__device__ __forceinline__ void direct_sums(const double4 *__restrict__ pos, int64_t n, int64_t cap,
+                                            __amdgpu_buffer_rsrc_t rl, uint32_t lane_off, uint32_t sub,
+                                            const double4 &P, const TypeFP64 &C64, float hf, float m2hinv,
+                                            float hhf, double hh, uint32_t cnt, uint32_t wmax, double &de,
+                                            double &f0, double &f1, double &f2) {
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc((void *)pos, 0, BUF ? (int)(uint32_t)(32 * (n + 1)) : 0, 0x00020000);
+  const uint32_t far = (uint32_t)n;
+  // Slots from a cell's count up to the longest list of its 64-cell group
+  // hold the stand-in n.  With one lane per cell in Morton order a wave is
+  // such a group, so every slot below wmax is a neighbour or the stand-in:
+  // no per-lane test.  Otherwise a lane's slots past its own count are
+  // replaced by the stand-in (with several lanes per cell the slot is first
+  // clamped into the cell's rows).
+  auto slot = [&](uint32_t kb) -> uint32_t {
+    if constexpr (MIXED && L == 1) {
+      return __builtin_amdgcn_raw_buffer_load_b32(rl, lane_off, (int)(256 * kb), 0);
+    } else {
+      const uint32_t k = kb + sub;
+      const uint32_t v =
+          __builtin_amdgcn_raw_buffer_load_b32(rl, lane_off + 256u * min(k, (uint32_t)cap - 1u), 0, 0);
+      return k < cnt ? v : far;
+    }
+  };
+  auto fetch = [&](uint32_t q) -> double4 { return gather<BUF>(pos, rs, q); };
+  cell_sums<MIXED, U, L, DIM3>(P, C64, hf, m2hinv, hhf, hh, cnt, wmax, far, slot, fetch, de, f0, f1, f2);
+}
+
+// descriptor over a list array from element e0 on (reads past its end return 0)
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t list_rsrc(const T *a, int64_t e0, int64_t elems) {
+  const int64_t rem = (int64_t)sizeof(T) * (elems - e0);
+  const uint32_t nrec = __builtin_amdgcn_readfirstlane((uint32_t)(rem > 0xFFFFFFFFll ? 0xFFFFFFFFll : rem));
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(a + e0), 0, (int)nrec, 0x00020000);
+}
+
+// Pair kernel without tiles: one workgroup per blockDim/L consecutive cells;
+// the FP64 form may rank a workgroup's cells by list length (balance).
+template <bool MIXED, int U, int L, bool DIM3, bool BUF>
 __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, const float *__restrict__ young_s,
                                              const int32_t *__restrict__ ctype_s, const TypeDev *__restrict__ types,
                                              int64_t n, int64_t cap, const uint32_t *__restrict__ nbr,
-                                             const uint32_t *__restrict__ count, const int32_t *__restrict__ qstar,
-                                             int dim, int balance, double *__restrict__ dens,
-                                             double *__restrict__ vout, double *__restrict__ fx,
-                                             double *__restrict__ fy, double *__restrict__ fz) {
+                                             int64_t nbr_elems, const uint32_t *__restrict__ count,
+                                             const int32_t *__restrict__ qstar, int balance,
+                                             double *__restrict__ dens, double *__restrict__ vout,
+                                             double *__restrict__ fx, double *__restrict__ fy,
+                                             double *__restrict__ fz) {
   const int64_t blk0 = tm_xcd_block() * blockDim.x;
   int64_t t = blk0 + threadIdx.x;
   if (!MIXED && balance) {
@@ -161,42 +321,23 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
   const int64_t cell = t / L;
   const uint32_t sub = (uint32_t)(t % L);
   const bool live = cell < n;
-  if (L == 1 && !live) return;
-  const int64_t p = live ? cell : n - 1;  // idle lanes of the last group stay for the reduction
-  const bool dim3 = dim == 3;
+  const int64_t p = live ? cell : n - 1;  // idle lanes take part in the wave-wide steps only
   const double4 P = pos[p];
   const TypeDev &T = types[ctype_s[p]];
-  const double h = T.h, sc = T.sc, csize = T.size, cW = T.cW, cF = T.cF;
-  const float hinv = T.hinv_f, cWf = T.cW_f, cFf = T.cF_f, scs = T.sc_over_size_f;
-  const float hf = (float)T.h, hhalf = 0.5f * (float)T.h;
+  const TypeFP64 C64{T.h, T.hinv, T.sc, T.size, T.sinv, T.cW, T.cF};
+  const float hf = (float)T.h, m2hinv = -2.0f * T.hinv_f, hhf = hf * hf;
   const double hh = T.h * T.h;
   const uint32_t cnt = live ? min((uint32_t)cap, count[p]) : 0u;
+  const uint32_t wmax = wave_max(cnt);
+  // The lists of this workgroup's cells lie in the slot groups from the one
+  // holding its first cell on: one descriptor based there serves the whole
+  // workgroup, a lane's slot k being at byte lane_off + 256 k.
+  const int64_t g0 = (blk0 / L) >> 6;
+  const __amdgpu_buffer_rsrc_t rl = list_rsrc(nbr, g0 * cap * 64, nbr_elems);
+  const uint32_t lane_off = (uint32_t)((((p >> 6) - g0) * cap * 64 + (p & 63)) * 4);
   double de = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  const uint32_t *col = nbr + tm_slot(p, 0, cap);  // slot k of this cell at col[64 k]
-  constexpr uint32_t STEP = (uint32_t)(U * L);
-  uint32_t nxt[U];
-#pragma unroll
-  for (int j = 0; j < U; ++j) {
-    const uint32_t k = sub + (uint32_t)(L * j);
-    nxt[j] = k < cnt ? col[(int64_t)k << 6] : (uint32_t)p;
-  }
-  for (uint32_t k0 = sub; k0 < cnt; k0 += STEP) {
-    double4 Q[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) Q[j] = pos[nxt[j]];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const uint32_t k = k0 + STEP + (uint32_t)(L * j);
-      nxt[j] = k < cnt ? col[(int64_t)k << 6] : (uint32_t)p;
-    }
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      if (k0 + (uint32_t)(L * j) < cnt) {
-        if (MIXED) pair_mixed(P, Q[j], hinv, hf, hhalf, hh, cWf, cFf, scs, dim3, de, f0, f1, f2);
-        else pair_fp64(P, Q[j], h, sc, csize, cW, cF, dim3, de, f0, f1, f2);
-      }
-    }
-  }
+  direct_sums<MIXED, U, L, DIM3, BUF>(pos, n, cap, rl, lane_off, sub, P, C64, hf, m2hinv, hhf, hh, cnt, wmax, de,
+                                      f0, f1, f2);
   if (L > 1) {
 #pragma unroll
     for (int o = 1; o < L; o <<= 1) {
@@ -205,15 +346,67 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
       f1 += __shfl_xor(f1, o, 64);
       f2 += __shfl_xor(f2, o, 64);
     }
-    if (sub != 0 || !live) return;
   }
-  const int32_t qs = qstar[p];
-  dens[p] = de;
-  if (MIXED) vout[p] = qs >= 0 ? (double)pot_fp32(P, young_s[p], pos[qs], young_s[qs], T.cut2) : 0.0;
-  else vout[p] = qs >= 0 ? pot_fp64(P, young_s[p], pos[qs], young_s[qs], h) : 0.0;
-  fx[p] = f0;
-  fy[p] = f1;
-  fz[p] = f2;
+  if (!live || sub != 0) return;
+  store_cell<MIXED>(p, P, T, de, f0, f1, f2, pos, young_s, qstar, dens, vout, fx, fy, fz);
+}
+
+// Pair kernel over LDS tiles: one workgroup of TM_TILE lanes per tile.  The
+// tile's distinct neighbours (and the far stand-in after them) are copied to
+// LDS once; lane r takes the cell of rank r and reads its 16-bit list, every
+// pair's record coming from LDS.  A tile with more distinct neighbours than
+// the LDS holds takes the 32-bit lists and global records instead.
+template <bool MIXED, int U>
+__global__ __launch_bounds__(TM_TILE) void k_sph_tile(
+    const double4 *__restrict__ pos, const float *__restrict__ young_s, const int32_t *__restrict__ ctype_s,
+    const TypeDev *__restrict__ types, int64_t n, int64_t cap, const uint32_t *__restrict__ nbr, int64_t nbr_elems,
+    const uint32_t *__restrict__ count, const int32_t *__restrict__ qstar, const uint32_t *__restrict__ tile_idx,
+    const uint32_t *__restrict__ tile_cnt, const uint8_t *__restrict__ tile_cell,
+    const uint16_t *__restrict__ nbr16, int64_t nbr16_elems, uint32_t lds_cap, double *__restrict__ dens,
+    double *__restrict__ vout, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
+  extern __shared__ double4 lp[];
+  const int64_t g = tm_xcd_block();
+  const uint32_t t = threadIdx.x;
+  const uint32_t m = __builtin_amdgcn_readfirstlane(tile_cnt[g]);
+  const bool staged = m < lds_cap;  // m records and the stand-in fit (TM_TILE_OVER never does)
+  if (staged) {
+    const uint32_t *ti = tile_idx + g * TM_TILE_MAX;
+    for (uint32_t i = t; i < m; i += TM_TILE) lp[i] = pos[ti[i]];
+    if (t == 0) lp[m] = make_double4(TM_FAR, TM_FAR, TM_FAR, 0.0);
+  }
+  const uint32_t off = staged ? (uint32_t)tile_cell[g * TM_TILE + t] : t;
+  const int64_t cell = g * TM_TILE + off;
+  const bool live = cell < n;
+  const int64_t p = live ? cell : n - 1;
+  const double4 P = pos[p];
+  const TypeDev &T = types[ctype_s[p]];
+  const TypeFP64 C64{T.h, T.hinv, T.sc, T.size, T.sinv, T.cW, T.cF};
+  const float hf = (float)T.h, m2hinv = -2.0f * T.hinv_f, hhf = hf * hf;
+  const double hh = T.h * T.h;
+  const uint32_t cnt = live ? min((uint32_t)cap, count[p]) : 0u;
+  const uint32_t wmax = wave_max(cnt);
+  double de = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  __syncthreads();
+  if (staged) {
+    // rank r's slot k at tm_slot(g TM_TILE + r, k, cap): rows of 64 ranks,
+    // 128 bytes; every slot below the wave's longest list is written (the
+    // padding holds m, the stand-in's place)
+    const __amdgpu_buffer_rsrc_t r16 = list_rsrc(nbr16, (g * (TM_TILE / 64)) * cap * 64, nbr16_elems);
+    const uint32_t lane_off = (uint32_t)((((t >> 6) * cap * 64) + (t & 63)) * 2);
+    auto slot = [&](uint32_t kb) -> uint32_t {
+      return __builtin_amdgcn_raw_buffer_load_b16(r16, lane_off, (int)(128 * kb), 0);
+    };
+    auto fetch = [&](uint32_t q) -> double4 { return lp[q]; };
+    cell_sums<MIXED, U, 1, true>(P, C64, hf, m2hinv, hhf, hh, cnt, wmax, m, slot, fetch, de, f0, f1, f2);
+  } else {
+    const int64_t g0 = g * (TM_TILE / 64);
+    const __amdgpu_buffer_rsrc_t rl = list_rsrc(nbr, g0 * cap * 64, nbr_elems);
+    const uint32_t lane_off = (uint32_t)((((p >> 6) - g0) * cap * 64 + (p & 63)) * 4);
+    direct_sums<MIXED, U, 1, true, true>(pos, n, cap, rl, lane_off, 0, P, C64, hf, m2hinv, hhf, hh, cnt, wmax, de,
+                                         f0, f1, f2);
+  }
+  if (!live) return;
+  store_cell<MIXED>(p, P, T, de, f0, f1, f2, pos, young_s, qstar, dens, vout, fx, fy, fz);
 }
 
 __global__ void k_unpermute(const int32_t *__restrict__ perm, int64_t n, const double *__restrict__ a0,
@@ -231,27 +424,55 @@ __global__ void k_unpermute(const int32_t *__restrict__ perm, int64_t n, const d
   out[5 * n + o] = a5[i];
 }
 
-template <bool MIXED, int U, int L>
-void launch_ul(tm_ctx *c) {
+template <bool MIXED, int U, int L, bool DIM3, bool BUF>
+void launch_k(tm_ctx *c) {
   const int64_t n = c->n;
   const int bs = c->sph_block;
-  k_sph<MIXED, U, L><<<tm_blocks(n * L, bs), bs, 0, c->stream>>>(
-      c->d_pos, c->d_young_s, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_qstar, c->dim,
-      c->sph_balance, c->d_dens, c->d_v, c->d_fx, c->d_fy, c->d_fz);
+  k_sph<MIXED, U, L, DIM3, BUF><<<tm_blocks(n * L, bs), bs, 0, c->stream>>>(
+      c->d_pos, c->d_young_s, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount,
+      c->d_qstar, c->sph_balance, c->d_dens, c->d_v, c->d_fx, c->d_fy, c->d_fz);
+}
+
+template <bool MIXED, int U>
+void launch_tile(tm_ctx *c) {
+  const int64_t n = c->n;
+  const int64_t nt = (n + TM_TILE - 1) / TM_TILE;
+  k_sph_tile<MIXED, U><<<(unsigned)nt, TM_TILE, sizeof(double4) * c->tile_lds, c->stream>>>(
+      c->d_pos, c->d_young_s, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount,
+      c->d_qstar, c->d_tile_idx, c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->cap_nbr16, c->tile_lds, c->d_dens,
+      c->d_v, c->d_fx, c->d_fy, c->d_fz);
 }
 
 template <bool MIXED, int L>
 void launch_u(tm_ctx *c) {
   switch (MIXED ? c->sph_unroll_mixed : c->sph_unroll_fp64) {
-    case 1: launch_ul<MIXED, 1, L>(c); break;
-    case 2: launch_ul<MIXED, 2, L>(c); break;
-    case 8: launch_ul<MIXED, 8, L>(c); break;
-    default: launch_ul<MIXED, 4, L>(c); break;
+    case 1: launch_k<MIXED, 1, L, true, true>(c); break;
+    case 2: launch_k<MIXED, 2, L, true, true>(c); break;
+    case 8: launch_k<MIXED, 8, L, true, true>(c); break;
+    default: launch_k<MIXED, 4, L, true, true>(c); break;
   }
 }
 
 template <bool MIXED>
 void launch_sph(tm_ctx *c) {
+  // 2-D colonies and colonies too large for 32-bit gather offsets take one
+  // general configuration; 3-D ones the tiles or the tuned direct one
+  const bool buf = 32 * (c->n + 1) <= (int64_t)0xFFFFFFFF;
+  if (c->dim != 3 || !buf) {
+    if (c->dim == 3) launch_k<MIXED, 4, 1, true, false>(c);
+    else if (buf) launch_k<MIXED, 4, 1, false, true>(c);
+    else launch_k<MIXED, 4, 1, false, false>(c);
+    return;
+  }
+  if (c->sph_tiles && c->tile_lds > 0) {
+    switch (MIXED ? c->sph_unroll_mixed : c->sph_unroll_fp64) {
+      case 1: launch_tile<MIXED, 1>(c); break;
+      case 2: launch_tile<MIXED, 2>(c); break;
+      case 8: launch_tile<MIXED, 8>(c); break;
+      default: launch_tile<MIXED, 4>(c); break;
+    }
+    return;
+  }
   if (!MIXED) {
     launch_u<false, 1>(c);  // the FP64 drop-in adds pairs in list order: one lane per cell
     return;

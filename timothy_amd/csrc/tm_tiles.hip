@@ -1,0 +1,192 @@
+// LDS tiles for the pair kernels, built from the neighbour lists.
+//
+// Cells are taken in tiles of TM_TILE consecutive cells of the Morton order.
+// A tile's neighbours overlap heavily (in the 2M-cell Gaussian colony about
+// 600 distinct neighbours serve 4,100 pairs per tile), so the pair kernel
+// stages the distinct ones in LDS once and reads every pair from there
+// instead of gathering 32-byte records through the vector memory path per
+// pair.  Per tile this kernel produces:
+//   tile_idx   the distinct neighbour indices, ascending (so the staging
+//              loads run along the Morton order), at most TM_TILE_MAX
+//   tile_cnt   their number, or TM_TILE_OVER when there are more
+//   tile_cell  a ranking of the tile's cells by list length, longest first,
+//              so each wave of the pair kernel takes cells of similar length
+//   nbr16      the lists as 16-bit positions in tile_idx, by rank, in the
+//              same order as the 32-bit lists (the FP64 sums keep the
+//              reference's order); each list is padded to the longest list
+//              of its wave with the position tile_cnt, where the pair kernel
+//              places the far stand-in record
+// The 32-bit lists stay as they are: the getters and over-full tiles use
+// them.
+#include "tm_internal.h"
+
+namespace {
+
+constexpr int TS = TM_TILE;
+constexpr int HASH = 4096;  // open-addressing set of distinct neighbours (> 2 TM_TILE_MAX)
+constexpr int SORTN = 2048;  // power of two >= TM_TILE_MAX
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 20; }  // 12 bits
+
+// exclusive prefix sum over the 256 threads of the workgroup
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *wsum) {
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t i = 0; i < w; ++i) base += wsum[i];
+  __syncthreads();
+  return base + inc - v;
+}
+
+__global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint32_t *__restrict__ nbr,
+                                              const uint32_t *__restrict__ count, uint32_t *__restrict__ tile_idx,
+                                              uint32_t *__restrict__ tile_cnt, uint8_t *__restrict__ tile_cell,
+                                              uint16_t *__restrict__ nbr16, unsigned int *__restrict__ flags) {
+  __shared__ uint32_t keys[HASH];
+  __shared__ uint16_t vals[HASH];
+  __shared__ uint32_t list[SORTN];
+  __shared__ uint32_t bins[TS];
+  __shared__ uint32_t rcnt[TS];
+  __shared__ uint8_t order[TS];
+  __shared__ uint32_t wsum[TS / 64];
+  __shared__ uint32_t distinct;
+  const int64_t g = blockIdx.x;
+  const uint32_t t = threadIdx.x;
+  const int64_t cell = g * TS + t;
+  const uint32_t cnt = cell < n ? min(count[cell], (uint32_t)cap) : 0u;
+  for (int i = t; i < HASH; i += TS) keys[i] = EMPTY;
+  bins[t] = 0;
+  if (t == 0) distinct = 0;
+  __syncthreads();
+
+  // rank the cells by list length, longest first (counting sort; the order
+  // among equal lengths is immaterial: each cell is still summed by one
+  // lane over its own list)
+  const uint32_t b = 255u - min(cnt, 255u);
+  atomicAdd(&bins[b], 1u);
+  __syncthreads();
+  const uint32_t start = block_scan(bins[t], wsum);
+  bins[t] = start;
+  __syncthreads();
+  const uint32_t rank = atomicAdd(&bins[b], 1u);
+  order[rank] = (uint8_t)t;
+  rcnt[rank] = cnt;
+
+  // distinct neighbours of the tile (stop counting past TM_TILE_MAX)
+  for (uint32_t k = 0; k < cnt; ++k) {
+    if (*(volatile uint32_t *)&distinct > TM_TILE_MAX) break;
+    const uint32_t q = nbr[tm_slot(cell, k, cap)];
+    uint32_t h = hslot(q);
+    for (;;) {
+      const uint32_t old = atomicCAS(&keys[h], EMPTY, q);
+      if (old == EMPTY) {
+        atomicAdd(&distinct, 1u);
+        break;
+      }
+      if (old == q) break;
+      h = (h + 1) & (HASH - 1);
+    }
+  }
+  __syncthreads();
+  const uint32_t m = distinct;
+  if (m > TM_TILE_MAX) {
+    if (t == 0) tile_cnt[g] = TM_TILE_OVER;
+    return;
+  }
+  // compact the set in slot order, then sort ascending
+  uint32_t mine = 0;
+  for (int i = 0; i < HASH / TS; ++i) mine += keys[t * (HASH / TS) + i] != EMPTY;
+  uint32_t w = block_scan(mine, wsum);
+  for (int i = 0; i < HASH / TS; ++i) {
+    const uint32_t k = keys[t * (HASH / TS) + i];
+    if (k != EMPTY) list[w++] = k;
+  }
+  uint32_t p2 = 64;
+  while (p2 < m) p2 <<= 1;
+  for (uint32_t i = m + t; i < p2; i += TS) list[i] = EMPTY;
+  __syncthreads();
+  for (uint32_t size = 2; size <= p2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = t; i < p2 / 2; i += TS) {
+        const uint32_t lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+        const uint32_t a = list[lo], c = list[hi];
+        if ((a > c) == ((lo & size) == 0)) {
+          list[lo] = c;
+          list[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // local position of every distinct neighbour
+  for (uint32_t i = t; i < m; i += TS) {
+    const uint32_t q = list[i];
+    uint32_t h = hslot(q);
+    while (keys[h] != q) h = (h + 1) & (HASH - 1);
+    vals[h] = (uint16_t)i;
+    tile_idx[g * TM_TILE_MAX + i] = q;
+  }
+  if (t == 0) {
+    tile_cnt[g] = m;
+    atomicMax(flags + 4, m);
+  }
+  __syncthreads();
+
+  // lists by rank: thread t writes rank t's list (rows of a wave coalesced)
+  const uint32_t off = order[t];
+  const int64_t src = g * TS + off;
+  const uint32_t rc = rcnt[t];
+  uint32_t pad = rc;  // longest list of this wave of ranks
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) pad = max(pad, (uint32_t)__shfl_xor((int)pad, o, 64));
+  tile_cell[g * TS + t] = (uint8_t)off;
+  const int64_t r = g * TS + t;
+  for (uint32_t k = 0; k < pad; ++k) {
+    uint16_t v = (uint16_t)m;
+    if (k < rc) {
+      const uint32_t q = nbr[tm_slot(src, k, cap)];
+      uint32_t h = hslot(q);
+      while (keys[h] != q) h = (h + 1) & (HASH - 1);
+      v = vals[h];
+    }
+    nbr16[tm_slot(r, k, cap)] = v;
+  }
+}
+
+__global__ void k_zero_one(unsigned int *p) { *p = 0; }
+
+}  // namespace
+
+tm_status tm_run_tiles(tm_ctx *c, bool check) {
+  const int64_t n = c->n;
+  if (!c->sph_tiles || n == 0) {
+    c->tile_lds = 0;
+    return TM_OK;
+  }
+  const int64_t nt = (n + TM_TILE - 1) / TM_TILE;
+  TM_TRY(tm_ensure(c, &c->d_tile_idx, c->cap_tile_idx, nt * TM_TILE_MAX, "tile indices"));
+  TM_TRY(tm_ensure(c, &c->d_tile_cnt, c->cap_tile_cnt, nt, "tile sizes"));
+  TM_TRY(tm_ensure(c, &c->d_tile_cell, c->cap_tile_cell, nt * TM_TILE, "tile ranks"));
+  TM_TRY(tm_ensure(c, &c->d_nbr16, c->cap_nbr16, tm_slot_elems(nt * TM_TILE, c->nbr_alloc), "tile lists"));
+  k_zero_one<<<1, 1, 0, c->stream>>>(c->d_flags + 4);
+  k_tiles<<<(unsigned)nt, TM_TILE, 0, c->stream>>>(n, c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_tile_idx,
+                                                    c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->d_flags);
+  TM_HIP(c, hipGetLastError());
+  if (check) {
+    TM_HIP(c, hipMemcpyAsync(c->h_flags + 4, c->d_flags + 4, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    TM_HIP(c, hipStreamSynchronize(c->stream));
+    // records staged per tile: the largest tile plus the far stand-in, rounded
+    c->tile_lds = (uint32_t)std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll);
+  } else if (c->tile_lds == 0) {
+    c->tile_lds = TM_TILE_MAX + 1;
+  }
+  return TM_OK;
+}
