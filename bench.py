@@ -65,7 +65,7 @@ def log(msg):
 
 
 def traffic_from_profiles():
-    p = os.path.join(ROOT, "profiles", "sph_mixed_c1_traffic.json")
+    p = os.path.join(ROOT, "profiles", "k_sph_tile_mixed_gaussian2M_traffic.json")
     if not os.path.exists(p):
         return None
     try:

@@ -111,28 +111,54 @@ def test_launch_shapes_invariant(tm, oracle_mod, ctx):
         got = ctx.neighbor_lists()
         assert np.array_equal(got[0], lists[0]) and np.array_equal(got[1], lists[1])
         assert np.array_equal(ctx.qstar(), exp_q)
-    for bs in (64, 256, 128):
-        for u in (1, 2, 8, 4):
-            for bal in (0, 1):
-                say(f"fp64 block {bs} unroll {u} balance {bal}")
-                ctx.set_param("sph_block", bs)
-                ctx.set_param("sph_unroll_fp64", u)
-                ctx.set_param("sph_balance", bal)
-                ctx.sph(tm.FP64)
+        ctx.sph(tm.FP64)
+        assert np.array_equal(ctx.results()["mindist"], ref["mindist"])
+    for tiles in (0, 1):
+        ctx.set_param("sph_tiles", tiles)
+        ctx.neighbors()
+        say(f"tiles {tiles}: staged records per tile {ctx.get_param('tile_lds')}")
+        for bs in (64, 256, 128):
+            for u in (1, 2, 8, 4):
+                for bal in (0, 1):
+                    say(f"fp64 tiles {tiles} block {bs} unroll {u} balance {bal}")
+                    ctx.set_param("sph_block", bs)
+                    ctx.set_param("sph_unroll_fp64", u)
+                    ctx.set_param("sph_balance", bal)
+                    ctx.sph(tm.FP64)
+                    r = ctx.results()
+                    for k in ("density", "v", "mindist", "force"):
+                        assert np.array_equal(r[k], ref[k]), k
+        for lanes in (2, 4, 8, 1):
+            for u in (1, 8, 4):
+                say(f"mixed tiles {tiles} lanes {lanes} unroll {u}")
+                ctx.set_param("sph_lanes", lanes)
+                ctx.set_param("sph_unroll_mixed", u)
+                ctx.sph(tm.MIXED)
                 r = ctx.results()
-                for k in ("density", "v", "mindist", "force"):
-                    assert np.array_equal(r[k], ref[k]), k
-    for lanes in (2, 4, 8, 1):
-        for u in (1, 8, 4):
-            say(f"mixed lanes {lanes} unroll {u}")
-            ctx.set_param("sph_lanes", lanes)
-            ctx.set_param("sph_unroll_mixed", u)
-            ctx.sph(tm.MIXED)
-            r = ctx.results()
-            assert np.array_equal(r["mindist"], ref["mindist"])
-            assert np.abs(r["force"] - ref["force"]).max() <= 1e-5 * np.abs(ref["force"]).max()
-            assert np.abs(r["density"] - ref["density"]).max() <= 1e-5 * np.abs(ref["density"]).max()
+                assert np.array_equal(r["mindist"], ref["mindist"])
+                assert np.abs(r["force"] - ref["force"]).max() <= 1e-5 * np.abs(ref["force"]).max()
+                assert np.abs(r["density"] - ref["density"]).max() <= 1e-5 * np.abs(ref["density"]).max()
     ctx.set_param("sph_unroll_mixed", 8)
+    ctx.set_param("sph_balance", 0)
+
+
+def test_tile_overflow_path(tm, oracle_mod, ctx):
+    """Cutoff 7 cell radii: a tile's distinct neighbours exceed what LDS
+    stages, so tiles fall back to the 32-bit lists inside the tile kernel;
+    FP64 stays bit-exact, mixed within its bound."""
+    O = oracle_mod
+    col = gen.ball_colony(6000, 99)
+    col["types"] = [dict(size=1.0, h=7.0, h2=49.0, h3=343.0, h4=2401.0)]
+    ctx.set_param("sph_tiles", 1)
+    res, oc, tree = run_both(tm, O, ctx, col, tm.FP64)
+    assert ctx.get_param("tile_lds") == 1537
+    ref = oracle_results(tree)
+    for k in ("density", "v", "mindist", "force"):
+        assert np.array_equal(res[k], ref[k]), k
+    ctx.sph(tm.MIXED)
+    r = ctx.results()
+    assert np.abs(r["force"] - ref["force"]).max() <= 1e-5 * np.abs(ref["force"]).max()
+    assert np.array_equal(r["mindist"], ref["mindist"])
 
 
 @pytest.mark.parametrize("name,make", CASES)

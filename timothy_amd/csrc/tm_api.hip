@@ -389,7 +389,7 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
   if (!c || !key) return TM_ERR_INVALID;
   if (!strcmp(key, "nbr_cap")) {
     if (v < 1 || v > 4096) return tm_fail(c, TM_ERR_INVALID, "nbr_cap in [1, 4096]");
-    c->nbr_cap = v;
+    c->nbr_cap = (v + 7) & ~7ll;  // lists are read in chunks of 8 slots
   } else if (!strcmp(key, "sph_block")) {
     if (v != 64 && v != 128 && v != 256) return tm_fail(c, TM_ERR_INVALID, "sph_block in {64,128,256}");
     c->sph_block = (int)v;

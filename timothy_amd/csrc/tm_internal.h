@@ -70,7 +70,7 @@ struct tm_ctx {
   int sph_unroll_mixed = 8;  // the same for the mixed kernel
   int nbr_block = 256;
   int sph_lanes = 1;
-  int sph_balance = 1;  // FP64 kernel: rank cells by list length inside each workgroup
+  int sph_balance = 0;  // FP64 kernel without tiles: rank cells by list length inside each workgroup
 
   // cells (reference order)
   int64_t n = 0, cap_n = 0;
@@ -133,7 +133,7 @@ struct tm_ctx {
   uint32_t *d_tile_idx = nullptr;   // per tile: distinct neighbours, ascending (TM_TILE_MAX slots)
   uint32_t *d_tile_cnt = nullptr;   // per tile: their number, or TM_TILE_OVER
   uint8_t *d_tile_cell = nullptr;   // per tile: cell offset of each rank (longest list first)
-  uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_slot(g*TM_TILE + r, k, nbr_alloc)
+  uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_nbr16_at(g*TM_TILE + r, k, nbr_alloc)
   int64_t cap_tile_idx = 0, cap_tile_cnt = 0, cap_tile_cell = 0, cap_nbr16 = 0;
   uint32_t tile_lds = 0;            // neighbour records the pair kernel stages per tile (0: no tiles built)
 
@@ -229,6 +229,11 @@ __host__ __device__ __forceinline__ int64_t tm_slot(int64_t p, int64_t k, int64_
   return (((p >> 6) * cap + k) << 6) | (p & 63);
 }
 inline int64_t tm_slot_elems(int64_t n, int64_t cap) { return ((n + 63) >> 6) * cap * 64; }
+// 16-bit tile lists: the slots of each rank in chunks of 8 (16 bytes), the
+// chunks of a group of 64 ranks side by side (cap a multiple of 8)
+__host__ __device__ __forceinline__ int64_t tm_nbr16_at(int64_t r, int64_t k, int64_t cap) {
+  return (((r >> 6) * cap + (k & ~7ll)) << 6) + ((r & 63) << 3) + (k & 7);
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
 // with its own L2.  Cell-ordered kernels remap so that XCD x runs one

@@ -105,6 +105,13 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   if (cnt > cap) atomicMax(flags + 1, cnt);
 }
 
+template <int BS>
+void launch_walk(tm_ctx *c) {
+  k_neighbors<BS><<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
+      c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes, c->d_colroot, c->d_aff, c->d_types, c->n, c->nbr_alloc,
+      c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_flags);
+}
+
 }  // namespace
 
 // grows the neighbour lists and repeats the walk when a cell overflowed them
@@ -129,21 +136,9 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     }
     c->nbr_alloc = c->nbr_cap;
     const int bs = c->nbr_block;
-    if (bs == 64)
-      k_neighbors<64><<<tm_blocks(n, 64), 64, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
-                                                               c->d_colroot, c->d_aff, c->d_types, n, c->nbr_alloc,
-                                                               c->d_nbr, c->d_ncount,
-                                                               c->d_mind, c->d_qstar, c->d_flags);
-    else if (bs == 128)
-      k_neighbors<128><<<tm_blocks(n, 128), 128, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
-                                                                  c->d_colroot, c->d_aff, c->d_types, n,
-                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount,
-                                                                  c->d_mind, c->d_qstar, c->d_flags);
-    else
-      k_neighbors<256><<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes,
-                                                                  c->d_colroot, c->d_aff, c->d_types, n,
-                                                                  c->nbr_alloc, c->d_nbr, c->d_ncount,
-                                                                  c->d_mind, c->d_qstar, c->d_flags);
+    if (bs == 64) launch_walk<64>(c);
+    else if (bs == 128) launch_walk<128>(c);
+    else launch_walk<256>(c);
     TM_HIP(c, hipGetLastError());
     if (!check) break;
     TM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));

@@ -353,9 +353,12 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
 
 // Pair kernel over LDS tiles: one workgroup of TM_TILE lanes per tile.  The
 // tile's distinct neighbours (and the far stand-in after them) are copied to
-// LDS once; lane r takes the cell of rank r and reads its 16-bit list, every
-// pair's record coming from LDS.  A tile with more distinct neighbours than
-// the LDS holds takes the 32-bit lists and global records instead.
+// LDS once, as two arrays of 16-byte halves (x, y and z, size: consecutive
+// records start in different bank groups, which halves bank conflicts
+// against 32-byte records).  Lane r takes the cell of rank r and reads its
+// 16-bit list 8 slots per load, every pair's record coming from LDS.  A tile
+// with more distinct neighbours than the LDS holds takes the 32-bit lists and
+// global records instead.
 template <bool MIXED, int U>
 __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
     const double4 *__restrict__ pos, const float *__restrict__ young_s, const int32_t *__restrict__ ctype_s,
@@ -364,15 +367,24 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
     const uint32_t *__restrict__ tile_cnt, const uint8_t *__restrict__ tile_cell,
     const uint16_t *__restrict__ nbr16, int64_t nbr16_elems, uint32_t lds_cap, double *__restrict__ dens,
     double *__restrict__ vout, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
-  extern __shared__ double4 lp[];
+  extern __shared__ double2 lds[];
+  double2 *const lxy = lds;
+  double2 *const lzs = lds + lds_cap;
   const int64_t g = tm_xcd_block();
   const uint32_t t = threadIdx.x;
   const uint32_t m = __builtin_amdgcn_readfirstlane(tile_cnt[g]);
   const bool staged = m < lds_cap;  // m records and the stand-in fit (TM_TILE_OVER never does)
   if (staged) {
     const uint32_t *ti = tile_idx + g * TM_TILE_MAX;
-    for (uint32_t i = t; i < m; i += TM_TILE) lp[i] = pos[ti[i]];
-    if (t == 0) lp[m] = make_double4(TM_FAR, TM_FAR, TM_FAR, 0.0);
+    for (uint32_t i = t; i < m; i += TM_TILE) {
+      const double4 q = pos[ti[i]];
+      lxy[i] = make_double2(q.x, q.y);
+      lzs[i] = make_double2(q.z, q.w);
+    }
+    if (t == 0) {
+      lxy[m] = make_double2(TM_FAR, TM_FAR);
+      lzs[m] = make_double2(TM_FAR, 0.0);
+    }
   }
   const uint32_t off = staged ? (uint32_t)tile_cell[g * TM_TILE + t] : t;
   const int64_t cell = g * TM_TILE + off;
@@ -388,16 +400,47 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
   double de = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
   __syncthreads();
   if (staged) {
-    // rank r's slot k at tm_slot(g TM_TILE + r, k, cap): rows of 64 ranks,
-    // 128 bytes; every slot below the wave's longest list is written (the
-    // padding holds m, the stand-in's place)
-    const __amdgpu_buffer_rsrc_t r16 = list_rsrc(nbr16, (g * (TM_TILE / 64)) * cap * 64, nbr16_elems);
-    const uint32_t lane_off = (uint32_t)((((t >> 6) * cap * 64) + (t & 63)) * 2);
-    auto slot = [&](uint32_t kb) -> uint32_t {
-      return __builtin_amdgcn_raw_buffer_load_b16(r16, lane_off, (int)(128 * kb), 0);
-    };
-    auto fetch = [&](uint32_t q) -> double4 { return lp[q]; };
-    cell_sums<MIXED, U, 1, true>(P, C64, hf, m2hinv, hhf, hh, cnt, wmax, m, slot, fetch, de, f0, f1, f2);
+    // every chunk below the wave's longest list is written (the padding
+    // holds m, the stand-in's place)
+    const __amdgpu_buffer_rsrc_t r16 = list_rsrc(nbr16, tm_nbr16_at(g * TM_TILE, 0, cap), nbr16_elems);
+    const uint32_t lane_off = (uint32_t)(tm_nbr16_at(t, 0, cap) * 2);
+    const uint32_t chunks = (wmax + 7) >> 3;
+    tm_u32x4 nx = {0u, 0u, 0u, 0u};
+    if (chunks) nx = __builtin_amdgcn_raw_buffer_load_b128(r16, lane_off, 0, 0);
+    for (uint32_t ch = 0; ch < chunks; ++ch) {
+      const tm_u32x4 cur = nx;
+      if (ch + 1 < chunks) nx = __builtin_amdgcn_raw_buffer_load_b128(r16, lane_off, (int)(1024 * (ch + 1)), 0);
+      const uint32_t idx[8] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16,
+                               cur.z & 0xFFFFu, cur.z >> 16, cur.w & 0xFFFFu, cur.w >> 16};
+      float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+      for (int j0 = 0; j0 < 8; j0 += U) {
+        double4 Q[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          const double2 a = lxy[idx[j0 + j]], b = lzs[idx[j0 + j]];
+          Q[j] = make_double4(a.x, a.y, b.x, b.y);
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+          if constexpr (MIXED) {
+            const MixedTerm mt = term_mixed(P, Q[j], m2hinv, hf, hhf, hh);
+            sw += mt.dw;
+            s0 = __builtin_fmaf(mt.coef, mt.dx, s0);
+            s1 = __builtin_fmaf(mt.coef, mt.dy, s1);
+            s2 = __builtin_fmaf(mt.coef, mt.dz, s2);
+          } else {
+            if (8 * ch + (uint32_t)(j0 + j) < cnt) pair_fp64(P, Q[j], C64, true, de, f0, f1, f2);
+          }
+        }
+      }
+      if constexpr (MIXED) {
+        de += (double)sw;
+        f0 += (double)s0;
+        f1 += (double)s1;
+        f2 += (double)s2;
+      }
+    }
   } else {
     const int64_t g0 = g * (TM_TILE / 64);
     const __amdgpu_buffer_rsrc_t rl = list_rsrc(nbr, g0 * cap * 64, nbr_elems);
@@ -437,7 +480,7 @@ template <bool MIXED, int U>
 void launch_tile(tm_ctx *c) {
   const int64_t n = c->n;
   const int64_t nt = (n + TM_TILE - 1) / TM_TILE;
-  k_sph_tile<MIXED, U><<<(unsigned)nt, TM_TILE, sizeof(double4) * c->tile_lds, c->stream>>>(
+  k_sph_tile<MIXED, U><<<(unsigned)nt, TM_TILE, 2 * sizeof(double2) * c->tile_lds, c->stream>>>(
       c->d_pos, c->d_young_s, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount,
       c->d_qstar, c->d_tile_idx, c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->cap_nbr16, c->tile_lds, c->d_dens,
       c->d_v, c->d_fx, c->d_fy, c->d_fz);
@@ -466,7 +509,7 @@ void launch_sph(tm_ctx *c) {
   }
   if (c->sph_tiles && c->tile_lds > 0) {
     switch (MIXED ? c->sph_unroll_mixed : c->sph_unroll_fp64) {
-      case 1: launch_tile<MIXED, 1>(c); break;
+      case 1:
       case 2: launch_tile<MIXED, 2>(c); break;
       case 8: launch_tile<MIXED, 8>(c); break;
       default: launch_tile<MIXED, 4>(c); break;

@@ -13,9 +13,11 @@
 //              so each wave of the pair kernel takes cells of similar length
 //   nbr16      the lists as 16-bit positions in tile_idx, by rank, in the
 //              same order as the 32-bit lists (the FP64 sums keep the
-//              reference's order); each list is padded to the longest list
-//              of its wave with the position tile_cnt, where the pair kernel
-//              places the far stand-in record
+//              reference's order), in chunks of 8 slots: slot k of rank r at
+//              nbr16_at(r, k, cap), so a lane reads 8 slots in one 16-byte
+//              load and a wave one 1 KB row; each list is padded to the
+//              longest list of its wave, rounded up to 8, with the position
+//              tile_cnt, where the pair kernel places the far stand-in
 // The 32-bit lists stay as they are: the getters and over-full tiles use
 // them.
 #include "tm_internal.h"
@@ -98,7 +100,10 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   __syncthreads();
   const uint32_t m = distinct;
   if (m > TM_TILE_MAX) {
-    if (t == 0) tile_cnt[g] = TM_TILE_OVER;
+    if (t == 0) {
+      tile_cnt[g] = TM_TILE_OVER;
+      atomicMax(flags + 4, m);
+    }
     return;
   }
   // compact the set in slot order, then sort ascending
@@ -149,7 +154,8 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   for (int o = 32; o >= 1; o >>= 1) pad = max(pad, (uint32_t)__shfl_xor((int)pad, o, 64));
   tile_cell[g * TS + t] = (uint8_t)off;
   const int64_t r = g * TS + t;
-  for (uint32_t k = 0; k < pad; ++k) {
+  const uint32_t pad8 = (pad + 7) & ~7u;  // <= cap, a multiple of 8
+  for (uint32_t k = 0; k < pad8; ++k) {
     uint16_t v = (uint16_t)m;
     if (k < rc) {
       const uint32_t q = nbr[tm_slot(src, k, cap)];
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
       while (keys[h] != q) h = (h + 1) & (HASH - 1);
       v = vals[h];
     }
-    nbr16[tm_slot(r, k, cap)] = v;
+    nbr16[tm_nbr16_at(r, k, cap)] = v;
   }
 }
 

@@ -7,7 +7,8 @@
         mean FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of one kernel, each
         scaled by what the same counter reads for the device-to-device copy
         kernel of BYTES bytes per dispatch in the same run (read factor =
-        BYTES / FETCH_SIZE, write factor = BYTES / WRITE_SIZE); prints HBM bytes
+        BYTES / FETCH_SIZE, write factor = BYTES / WRITE_SIZE, over the copy
+        dispatches of at least half the largest one); prints HBM bytes
         per launch and, with --json, writes them for the benchmark to report.
 
 Reads the SQLite databases rocprofv3 writes (*_results.db); nothing here
@@ -88,6 +89,9 @@ def traffic(fetch_dir, write_dir, kernel, copy_bytes):
     _, vw = pick(w, kernel)
     _, cf = pick(f, "__amd_rocclr_copyBuffer")
     _, cw = pick(w, "__amd_rocclr_copyBuffer")
+    # the calibration copy's dispatches only (other small copies in the run are dropped)
+    cf = [v for v in cf if v >= 0.5 * max(cf)]
+    cw = [v for v in cw if v >= 0.5 * max(cw)]
     rf = copy_bytes / (_mean(cf) * 1024.0)
     wf = copy_bytes / (_mean(cw) * 1024.0)
     rb = _mean(vf) * 1024.0 * rf
