@@ -18,10 +18,10 @@ captured in a HIP graph and replayed; per-colony mixed-vs-FP64 errors.
 of cell volume onto 512^3 FP64, FP32 field gradient and interpolation of a
 seeded smooth field on 512^3.
 
-roofline: algorithmic bytes of one SPH launch, B = 96 N + 4 P (per cell:
-x,y,z,size 32, ctype 4, list offset 8, young 4 read; density 8, force 24, v 8,
-mindist 8 written; per pair one u32 index), over the mean launch time, against
-MI355X's 8 TB/s.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE /
+roofline: algorithmic bytes of one SPH launch, B = 76 N + 4 P (per cell:
+x,y,z,size 32, ctype 4, list offset 8 read; density 8, force 24 written; per
+pair one u32 index; v and mindist are produced by the neighbour walk), over
+the mean launch time, against MI355X's 8 TB/s.  traffic: HBM bytes per launch from rocprofv3 FETCH_SIZE /
 WRITE_SIZE counters when a committed summary exists (profiles/), else null.
 
 cpu_baseline: the CPU restatement of the reference path (pointer PR-octree,
@@ -187,7 +187,7 @@ def interp_errors(tm, gen, ctx, col, field_seed=5):
 
 
 def time_sph(tm, ctx, a, n, P):
-    B = 96.0 * n + 4.0 * P
+    B = 76.0 * n + 4.0 * P
     out = {}
     for name, prec in (("mixed", tm.MIXED), ("fp64", tm.FP64)):
         if a.only not in ("both", name):

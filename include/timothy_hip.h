@@ -9,8 +9,9 @@
  *                                   29-bit location codes, Morton sort,
  *                                   PR-octree leaf octants, radix tree
  *   tm_neighbors                    tree walk -> neighbour lists, mindist,
- *                                   first-visited candidate (for v)
- *   tm_sph                          density, v, forces: TM_FP64 (bit-exact
+ *                                   the last candidate visited and v (the
+ *                                   potential with it, FP64), LDS tiles
+ *   tm_sph                          density and forces: TM_FP64 (bit-exact
  *                                   drop-in) or TM_MIXED
  *   tm_cic_deposit / tm_field_gradient / tm_cic_interp
  *                                   cells <-> grid transfers
@@ -127,7 +128,9 @@ tm_status tm_bind_device(tm_ctx *ctx, int64_t ncells, int32_t ncolonies, const i
 
 /* -------- path stages -------- */
 tm_status tm_build(tm_ctx *ctx);
+/* neighbour lists, mindist, v (src/potential.c:87-138 walk 1) */
 tm_status tm_neighbors(tm_ctx *ctx);
+/* density and forces over the lists (the pair sums of both walks) */
 tm_status tm_sph(tm_ctx *ctx, tm_precision precision);
 /* build + neighbors + sph */
 tm_status tm_step(tm_ctx *ctx, tm_precision precision);

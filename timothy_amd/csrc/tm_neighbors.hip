@@ -13,6 +13,7 @@
 //           correctly rounded, so this is the minimum of the rounded roots)
 //   qstar   the last candidate visited = minimum key, the one whose potential
 //           the reference leaves in v (src/potential.c:124 assigns, not adds)
+//   v       that potential, in the reference's FP64 expressions
 // The cutoff test d <= h is evaluated exactly as d2 <= cut2, cut2 being the
 // largest double whose correctly rounded root is <= h.
 #include "tm_internal.h"
@@ -23,16 +24,37 @@ __device__ __forceinline__ uint32_t box_code(double v, double shift, double scal
   return (uint32_t)((v - shift) / scale * 536870912.0);
 }
 
+// potential() (src/potential.c:39-78) without its density/mindist side effects,
+// in the reference's expressions and order
+__device__ __forceinline__ double pot_fp64(double4 P, float yp, double4 Q, float yq, double h) {
+  const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+  const double dist = sqrt((dx * dx + dy * dy) + dz * dz);
+  if (!(dist <= h)) return 0.0;
+  const double s1 = P.w, s2 = Q.w;
+  const double xc = s1 + s2 - dist;
+  if (xc <= 0.0) return 0.0;
+  const double poisson = 0.33;
+  const double D = 0.75 * ((1.0 - poisson * poisson) / (double)yp + (1.0 - poisson * poisson / (double)yq));
+  const double r01 = (s1 * s1 - s2 * s2 + dist * dist) / (2.0 * dist);
+  const double r02 = dist - r01;
+  const double area = M_PI * ((s1 * s1 * (s1 - r01) - (s1 * s1 * s1 - r01 * r01 * r01) / 3.0) +
+                              (s2 * s2 * (s2 - r02) - (s2 * s2 * s2 - r02 * r02 * r02) / 3.0));
+  // pow(xc, 5/2) with integer 5/2 == 2 is xc*xc
+  return (2.0 * (xc * xc) / (5.0 * D)) * sqrt((s1 * s2) / (s1 + s2)) + area * 0.1;
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
                                                   const int32_t *__restrict__ ctype_s,
+                                                  const float *__restrict__ young_s,
                                                   const NodeRec *__restrict__ nodes,
                                                   const int32_t *__restrict__ colroot,
                                                   const double4 *__restrict__ aff,
                                                   const TypeDev *__restrict__ types, int64_t n, int64_t cap,
                                                   uint32_t *__restrict__ nbr,
                                                   uint32_t *__restrict__ count, double *__restrict__ mindist,
-                                                  int32_t *__restrict__ qstar, unsigned int *__restrict__ flags) {
+                                                  int32_t *__restrict__ qstar, double *__restrict__ vout,
+                                                  unsigned int *__restrict__ flags) {
   const int64_t cell = tm_xcd_block() * BS + threadIdx.x;
   const bool live = cell < n;
   const int64_t p = live ? cell : n - 1;  // idle lanes only join the padding step below
@@ -51,6 +73,7 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   uint32_t node = live ? (uint32_t)colroot[cid] : TM_END;
   uint32_t cnt = 0;
   int32_t qs = -1;
+  double4 Qs = P;  // record of the last candidate visited
   double md2 = 1.0 / 0.0;
   // a well-formed walk visits each node at most once; the budget only stops a
   // malformed tree from looping
@@ -64,6 +87,7 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
     const double d2 = (dx * dx + dy * dy) + dz * dz;
     md2 = d2 < md2 ? d2 : md2;
     qs = (int32_t)q;
+    Qs = Q;
     if (d2 <= cut2) {
       if (cnt < cap) nbr[tm_slot(p, cnt, cap)] = q;
       ++cnt;
@@ -102,14 +126,17 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   count[p] = cnt;
   mindist[p] = qs >= 0 ? sqrt(md2) : 1.7976931348623157e308;
   qstar[p] = qs;
+  // v: the potential with the last candidate visited (src/potential.c:124
+  // assigns it per candidate; 0 beyond the cutoff and with no candidate)
+  vout[p] = qs >= 0 ? pot_fp64(P, young_s[p], Qs, young_s[qs], h) : 0.0;
   if (cnt > cap) atomicMax(flags + 1, cnt);
 }
 
 template <int BS>
 void launch_walk(tm_ctx *c) {
   k_neighbors<BS><<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
-      c->d_pos, c->d_code, c->d_ctype_s, c->d_nodes, c->d_colroot, c->d_aff, c->d_types, c->n, c->nbr_alloc,
-      c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_flags);
+      c->d_pos, c->d_code, c->d_ctype_s, c->d_young_s, c->d_nodes, c->d_colroot, c->d_aff, c->d_types, c->n,
+      c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_v, c->d_flags);
 }
 
 }  // namespace

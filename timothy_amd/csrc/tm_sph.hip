@@ -1,65 +1,22 @@
-// SPH stage: density, v and kernel-gradient force per cell over its
-// neighbour list, one lane per cell, pairs in the reference's accumulation
-// order (descending Morton key).
+// SPH stage: density and kernel-gradient force per cell over its neighbour
+// list, pairs in the reference's accumulation order (descending Morton key).
+// v and mindist come from the neighbour walk, which visits the candidates.
 //
 // TM_FP64 restates computepotential/computegradient's pair arithmetic
 // (src/potential.c:39-78,193-202, src/inline.h:40-83) operation by operation:
-// no contraction (built with -ffp-contract=off), IEEE division and sqrt, the
-// kernel value rounded to float before it is added (sphkernel returns float),
-// so results equal the CPU path bit for bit.  Density and force share one
-// distance and one u = d/h, which the reference computes twice identically.
+// no contraction (built with -ffp-contract=off), correctly rounded division
+// and sqrt, the kernel value rounded to float before it is added (sphkernel
+// returns float), so results equal the CPU path bit for bit.  Density and
+// force share one distance and one u = d/h, which the reference computes
+// twice identically.
 //
 // TM_MIXED keeps the exact neighbour set (built with the FP64 predicate),
 // forms each pair's separation in FP64 and rounds it once to FP32, evaluates
-// the kernel polynomials, potential and force vector in FP32 with the
-// hardware reciprocal square root, and accumulates each cell's sums in FP64.
-// Its v uses the closed form of the overlap-lens volume (algebraically equal
-// to the reference's cap-difference expression, free of its cancellation).
+// the kernel polynomials and force vector in FP32 with the hardware
+// reciprocal square root, and accumulates each cell's sums in FP64.
 #include "tm_internal.h"
 
 namespace {
-
-// src/potential.c:39-78 without the density/mindist side effects
-__device__ __forceinline__ double pot_fp64(double4 P, float yp, double4 Q, float yq, double h) {
-  const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-  const double dist = sqrt((dx * dx + dy * dy) + dz * dz);
-  if (!(dist <= h)) return 0.0;
-  const double s1 = P.w, s2 = Q.w;
-  const double xc = s1 + s2 - dist;
-  if (xc <= 0.0) return 0.0;
-  const double poisson = 0.33;
-  const double D = 0.75 * ((1.0 - poisson * poisson) / (double)yp + (1.0 - poisson * poisson / (double)yq));
-  const double r01 = (s1 * s1 - s2 * s2 + dist * dist) / (2.0 * dist);
-  const double r02 = dist - r01;
-  const double area = M_PI * ((s1 * s1 * (s1 - r01) - (s1 * s1 * s1 - r01 * r01 * r01) / 3.0) +
-                              (s2 * s2 * (s2 - r02) - (s2 * s2 * s2 - r02 * r02 * r02) / 3.0));
-  // pow(xc, 5/2) with integer 5/2 == 2 is xc*xc
-  return (2.0 * (xc * xc) / (5.0 * D)) * sqrt((s1 * s2) / (s1 + s2)) + area * 0.1;
-}
-
-// The overlap xc = (s1+s2) - d is formed as ((s1+s2)^2 - d^2) / ((s1+s2) + d)
-// with the difference of squares in FP64, so it keeps its relative accuracy
-// for barely touching cells.
-__device__ __forceinline__ float pot_fp32(double4 P, float yp, double4 Q, float yq, double cut2) {
-  const double ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
-  const double d2d = (ex * ex + ey * ey) + ez * ez;
-  if (!(d2d <= cut2)) return 0.0f;
-  const float d2 = (float)d2d;
-  const float d = d2 * __builtin_amdgcn_rsqf(d2);
-  const float s1 = (float)P.w, s2 = (float)Q.w;
-  const double sum = P.w + Q.w;
-  const double gap2 = sum * sum - d2d;
-  if (gap2 <= 0.0) return 0.0f;
-  const float xc = (float)gap2 * __builtin_amdgcn_rcpf((s1 + s2) + d);
-  const float pp = 0.1089f;
-  const float D = 0.75f * ((1.0f - pp) * __builtin_amdgcn_rcpf(yp) + (1.0f - pp * __builtin_amdgcn_rcpf(yq)));
-  const float ds = s1 - s2;
-  // lens volume pi xc^2 (d^2 + 2 d (s1+s2) - 3 (s1-s2)^2) / (12 d)
-  const float lens = 3.14159265f * xc * xc * (d2 + 2.0f * d * (s1 + s2) - 3.0f * ds * ds) *
-                     __builtin_amdgcn_rcpf(12.0f * d);
-  const float red = s1 * s2 * __builtin_amdgcn_rcpf(s1 + s2);
-  return (0.4f * xc * xc) * __builtin_amdgcn_rcpf(D) * (red * __builtin_amdgcn_rsqf(red)) + 0.1f * lens;
-}
 
 // a / b correctly rounded from y = RN(1/b): q0 = RN(a y) is within one ulp of
 // a / b, the remainder a - b q0 is exact in one FMA, and RN(q0 + r y) is the
@@ -211,27 +168,14 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-// The cell's outputs: density, v (the potential with its first-visited
-// candidate qstar, src/potential.c:124) and force.
+// the per-type constants of the mixed sums, applied once per cell
 template <bool MIXED>
-__device__ __forceinline__ void store_cell(int64_t p, const double4 &P, const TypeDev &T, double de, double f0,
-                                           double f1, double f2, const double4 *__restrict__ pos,
-                                           const float *__restrict__ young_s, const int32_t *__restrict__ qstar,
-                                           double *__restrict__ dens, double *__restrict__ vout,
-                                           double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
-  const int32_t qs = qstar[p];
+__device__ __forceinline__ void finish_cell(const TypeDev &T, double &de, double &f0, double &f1, double &f2) {
   if (MIXED) {
-    dens[p] = de * (T.sc / T.size * T.cW);
-    vout[p] = qs >= 0 ? (double)pot_fp32(P, young_s[p], pos[qs], young_s[qs], T.cut2) : 0.0;
-    fx[p] = f0 * T.cF;
-    fy[p] = f1 * T.cF;
-    fz[p] = f2 * T.cF;
-  } else {
-    dens[p] = de;
-    vout[p] = qs >= 0 ? pot_fp64(P, young_s[p], pos[qs], young_s[qs], T.h) : 0.0;
-    fx[p] = f0;
-    fy[p] = f1;
-    fz[p] = f2;
+    de *= T.sc / T.size * T.cW;
+    f0 *= T.cF;
+    f1 *= T.cF;
+    f2 *= T.cF;
   }
 }
 
@@ -279,14 +223,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t list_rsrc(const T *a, int64_t 
 // Pair kernel without tiles: one workgroup per blockDim/L consecutive cells;
 // the FP64 form may rank a workgroup's cells by list length (balance).
 template <bool MIXED, int U, int L, bool DIM3, bool BUF>
-__global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, const float *__restrict__ young_s,
-                                             const int32_t *__restrict__ ctype_s, const TypeDev *__restrict__ types,
-                                             int64_t n, int64_t cap, const uint32_t *__restrict__ nbr,
-                                             int64_t nbr_elems, const uint32_t *__restrict__ count,
-                                             const int32_t *__restrict__ qstar, int balance,
-                                             double *__restrict__ dens, double *__restrict__ vout,
-                                             double *__restrict__ fx, double *__restrict__ fy,
-                                             double *__restrict__ fz) {
+__global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, const int32_t *__restrict__ ctype_s,
+                                             const TypeDev *__restrict__ types, int64_t n, int64_t cap,
+                                             const uint32_t *__restrict__ nbr, int64_t nbr_elems,
+                                             const uint32_t *__restrict__ count, int balance,
+                                             double *__restrict__ dens, double *__restrict__ fx,
+                                             double *__restrict__ fy, double *__restrict__ fz) {
   const int64_t blk0 = tm_xcd_block() * blockDim.x;
   int64_t t = blk0 + threadIdx.x;
   if (!MIXED && balance) {
@@ -348,7 +290,11 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
     }
   }
   if (!live || sub != 0) return;
-  store_cell<MIXED>(p, P, T, de, f0, f1, f2, pos, young_s, qstar, dens, vout, fx, fy, fz);
+  finish_cell<MIXED>(T, de, f0, f1, f2);
+  dens[p] = de;
+  fx[p] = f0;
+  fy[p] = f1;
+  fz[p] = f2;
 }
 
 // Pair kernel over LDS tiles: one workgroup of TM_TILE lanes per tile.  The
@@ -361,12 +307,12 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
 // global records instead.
 template <bool MIXED, int U>
 __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
-    const double4 *__restrict__ pos, const float *__restrict__ young_s, const int32_t *__restrict__ ctype_s,
-    const TypeDev *__restrict__ types, int64_t n, int64_t cap, const uint32_t *__restrict__ nbr, int64_t nbr_elems,
-    const uint32_t *__restrict__ count, const int32_t *__restrict__ qstar, const uint32_t *__restrict__ tile_idx,
-    const uint32_t *__restrict__ tile_cnt, const uint8_t *__restrict__ tile_cell,
-    const uint16_t *__restrict__ nbr16, int64_t nbr16_elems, uint32_t lds_cap, double *__restrict__ dens,
-    double *__restrict__ vout, double *__restrict__ fx, double *__restrict__ fy, double *__restrict__ fz) {
+    const double4 *__restrict__ pos, const int32_t *__restrict__ ctype_s, const TypeDev *__restrict__ types,
+    int64_t n, int64_t cap, const uint32_t *__restrict__ nbr, int64_t nbr_elems, const uint32_t *__restrict__ count,
+    const uint32_t *__restrict__ tile_idx, const uint32_t *__restrict__ tile_cnt,
+    const uint8_t *__restrict__ tile_cell, const uint16_t *__restrict__ nbr16, int64_t nbr16_elems,
+    uint32_t lds_cap, double *__restrict__ dens, double *__restrict__ fx, double *__restrict__ fy,
+    double *__restrict__ fz) {
   extern __shared__ double2 lds[];
   double2 *const lxy = lds;
   double2 *const lzs = lds + lds_cap;
@@ -415,6 +361,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
       float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
       for (int j0 = 0; j0 < 8; j0 += U) {
+        if (j0 > 0 && 8 * ch + (uint32_t)j0 >= wmax) break;  // past every list of the wave
         double4 Q[U];
 #pragma unroll
         for (int j = 0; j < U; ++j) {
@@ -448,8 +395,21 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
     direct_sums<MIXED, U, 1, true, true>(pos, n, cap, rl, lane_off, 0, P, C64, hf, m2hinv, hhf, hh, cnt, wmax, de,
                                          f0, f1, f2);
   }
-  if (!live) return;
-  store_cell<MIXED>(p, P, T, de, f0, f1, f2, pos, young_s, qstar, dens, vout, fx, fy, fz);
+  // Results go out in cell order: each lane parks its cell's sums in LDS
+  // (the staged records are no longer needed) and lane t writes cell t, so
+  // every store of a wave is one contiguous run.
+  finish_cell<MIXED>(T, de, f0, f1, f2);
+  __syncthreads();
+  lxy[2 * off] = make_double2(de, f0);
+  lxy[2 * off + 1] = make_double2(f1, f2);
+  __syncthreads();
+  const int64_t c = g * TM_TILE + t;
+  if (c >= n) return;
+  const double2 a = lxy[2 * t], b = lxy[2 * t + 1];
+  dens[c] = a.x;
+  fx[c] = a.y;
+  fy[c] = b.x;
+  fz[c] = b.y;
 }
 
 __global__ void k_unpermute(const int32_t *__restrict__ perm, int64_t n, const double *__restrict__ a0,
@@ -472,8 +432,8 @@ void launch_k(tm_ctx *c) {
   const int64_t n = c->n;
   const int bs = c->sph_block;
   k_sph<MIXED, U, L, DIM3, BUF><<<tm_blocks(n * L, bs), bs, 0, c->stream>>>(
-      c->d_pos, c->d_young_s, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount,
-      c->d_qstar, c->sph_balance, c->d_dens, c->d_v, c->d_fx, c->d_fy, c->d_fz);
+      c->d_pos, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount, c->sph_balance,
+      c->d_dens, c->d_fx, c->d_fy, c->d_fz);
 }
 
 template <bool MIXED, int U>
@@ -481,9 +441,8 @@ void launch_tile(tm_ctx *c) {
   const int64_t n = c->n;
   const int64_t nt = (n + TM_TILE - 1) / TM_TILE;
   k_sph_tile<MIXED, U><<<(unsigned)nt, TM_TILE, 2 * sizeof(double2) * c->tile_lds, c->stream>>>(
-      c->d_pos, c->d_young_s, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount,
-      c->d_qstar, c->d_tile_idx, c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->cap_nbr16, c->tile_lds, c->d_dens,
-      c->d_v, c->d_fx, c->d_fy, c->d_fz);
+      c->d_pos, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount, c->d_tile_idx,
+      c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->cap_nbr16, c->tile_lds, c->d_dens, c->d_fx, c->d_fy, c->d_fz);
 }
 
 template <bool MIXED, int L>

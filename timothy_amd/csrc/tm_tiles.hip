@@ -190,7 +190,9 @@ tm_status tm_run_tiles(tm_ctx *c, bool check) {
     TM_HIP(c, hipMemcpyAsync(c->h_flags + 4, c->d_flags + 4, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
     TM_HIP(c, hipStreamSynchronize(c->stream));
     // records staged per tile: the largest tile plus the far stand-in, rounded
-    c->tile_lds = (uint32_t)std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll);
+    // (at least TM_TILE: the pair kernel also parks a tile's sums there)
+    c->tile_lds = (uint32_t)std::max<int64_t>(
+        TM_TILE, std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll));
   } else if (c->tile_lds == 0) {
     c->tile_lds = TM_TILE_MAX + 1;
   }
