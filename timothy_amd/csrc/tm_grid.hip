@@ -11,8 +11,9 @@
 //   d = (x - cicCoord) / resolution, t = 1 - d, corner weight a*d + (1-a)*t
 // so corner indices match bit for bit and the interpolation (one lane per
 // cell, corners in the reference's ax, ay, az order) matches bit for bit.
-// The deposit sums into each node with FP64 atomics: same terms as the
-// reference, in a different order (<= ~1e-15 relative for positive weights).
+// The deposit sums the reference's terms into each node through LDS partial
+// sums and FP64 atomics: the same terms in a different order (<= ~1e-15
+// relative for positive weights).
 #include "tm_internal.h"
 
 namespace {
@@ -59,36 +60,91 @@ __global__ void k_fill_f64(double *p, int64_t n) {
   for (; i < n; i += stride) p[i] = 0.0;
 }
 
-// cells in Morton order, so neighbouring lanes hit neighbouring nodes
+// Deposit, one workgroup per 256 cells of the Morton order.  Those cells lie
+// in a small region, so their corners usually fit a box of at most DEP_LDS
+// nodes: the workgroup adds its terms into that box in LDS, then adds each
+// touched node to the grid with one global atomic (a few per cell instead of
+// eight).  A workgroup whose box is larger adds its terms to the grid
+// directly.
+constexpr int DEP_LDS = 4096;  // 32 KB of FP64 partial sums
+
 __global__ __launch_bounds__(256) void k_deposit(const double4 *__restrict__ pos, const int32_t *__restrict__ phase_s,
                                                  int64_t n, GridDev g, int weight_mode,
                                                  double *__restrict__ prod) {
+  __shared__ double acc[DEP_LDS];
+  __shared__ int box[6];  // min x, y, z of the cells' first corners; max x, y, z
   const int64_t i = tm_xcd_block() * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double4 P = pos[i];
-  double wt;
-  if (weight_mode == 1) {
-    wt = ((4.0 / 3.0) * M_PI) * (P.w * P.w * P.w);
-  } else {
-    const int ph = phase_s[i];
-    if (ph == 5) return;           // necrotic: no deposit
-    wt = ph == 0 ? 0.75 : 1.0;     // G0: lower consumption
+  if (threadIdx.x < 3) box[threadIdx.x] = 0x7FFFFFFF;
+  else if (threadIdx.x < 6) box[threadIdx.x] = -0x7FFFFFFF;
+  double wt = 0.0;
+  bool dep = false;
+  double4 P = make_double4(0, 0, 0, 0);
+  if (i < n) {
+    P = pos[i];
+    if (weight_mode == 1) {
+      wt = ((4.0 / 3.0) * M_PI) * (P.w * P.w * P.w);
+      dep = true;
+    } else {
+      const int ph = phase_s[i];
+      dep = ph != 5;                 // necrotic: no deposit
+      wt = ph == 0 ? 0.75 : 1.0;     // G0: lower consumption
+    }
   }
-  int64_t ci[3];
-  double d[3], t[3];
-  cic_setup(g, P.x, P.y, P.z, ci, d, t);
+  int64_t ci[3] = {0, 0, 0};
+  double d[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+  if (dep) cic_setup(g, P.x, P.y, P.z, ci, d, t);
+  // the box only matters for corners in range: clamp to one node beyond it
+  int cl[3];
 #pragma unroll
-  for (int ax = 0; ax < 2; ++ax)
+  for (int a = 0; a < 3; ++a) cl[a] = (int)max(min(ci[a], g.hi[a] + 1), g.lo[a] - 1);
+  __syncthreads();
+  if (dep) {
 #pragma unroll
-    for (int ay = 0; ay < 2; ++ay)
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&box[a], cl[a]);
+      atomicMax(&box[3 + a], cl[a]);
+    }
+  }
+  __syncthreads();
+  const bool any = box[0] <= box[3];  // some lane deposits
+  const int b0 = box[0], b1 = box[1], b2 = box[2];
+  // extent of the corners ci and ci + 1
+  const int e0 = any ? box[3] - b0 + 2 : 0, e1 = any ? box[4] - b1 + 2 : 0, e2 = any ? box[5] - b2 + 2 : 0;
+  const bool local = any && (int64_t)e0 * e1 * e2 <= DEP_LDS;
+  if (local) {
+    const int vol = e0 * e1 * e2;
+    for (int k = threadIdx.x; k < vol; k += 256) acc[k] = 0.0;
+    __syncthreads();
+  }
+  if (dep) {
 #pragma unroll
-      for (int az = 0; az < 2; ++az) {
-        if (!cic_inrange(g, ci, ax, ay, az)) continue;
-        const int64_t gx = ci[0] + ax - g.lo[0], gy = ci[1] + ay - g.lo[1], gz = ci[2] + az - g.lo[2];
-        const double val = wt * (ax * d[0] + (1 - ax) * t[0]) * (ay * d[1] + (1 - ay) * t[1]) *
-                           (az * d[2] + (1 - az) * t[2]);
-        unsafeAtomicAdd(prod + (gx * g.ls[1] + gy) * g.ls[2] + gz, val);
-      }
+    for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+      for (int ay = 0; ay < 2; ++ay)
+#pragma unroll
+        for (int az = 0; az < 2; ++az) {
+          if (!cic_inrange(g, ci, ax, ay, az)) continue;
+          const double val = wt * (ax * d[0] + (1 - ax) * t[0]) * (ay * d[1] + (1 - ay) * t[1]) *
+                             (az * d[2] + (1 - az) * t[2]);
+          if (local) {
+            const int lx = (int)ci[0] + ax - b0, ly = (int)ci[1] + ay - b1, lz = (int)ci[2] + az - b2;
+            atomicAdd(&acc[(lx * e1 + ly) * e2 + lz], val);
+          } else {
+            const int64_t gx = ci[0] + ax - g.lo[0], gy = ci[1] + ay - g.lo[1], gz = ci[2] + az - g.lo[2];
+            unsafeAtomicAdd(prod + (gx * g.ls[1] + gy) * g.ls[2] + gz, val);
+          }
+        }
+  }
+  if (!local) return;
+  __syncthreads();
+  const int vol = e0 * e1 * e2;
+  for (int k = threadIdx.x; k < vol; k += 256) {
+    const double v = acc[k];
+    if (v == 0.0) continue;  // untouched (only in-range corners were added)
+    const int lz = k % e2, ly = (k / e2) % e1, lx = k / (e1 * e2);
+    const int64_t gx = b0 + lx - g.lo[0], gy = b1 + ly - g.lo[1], gz = b2 + lz - g.lo[2];
+    unsafeAtomicAdd(prod + (gx * g.ls[1] + gy) * g.ls[2] + gz, v);
+  }
 }
 
 template <typename F>
