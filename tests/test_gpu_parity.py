@@ -138,7 +138,7 @@ def test_launch_shapes_invariant(tm, oracle_mod, ctx):
                 assert np.array_equal(r["mindist"], ref["mindist"])
                 assert np.abs(r["force"] - ref["force"]).max() <= 1e-5 * np.abs(ref["force"]).max()
                 assert np.abs(r["density"] - ref["density"]).max() <= 1e-5 * np.abs(ref["density"]).max()
-    ctx.set_param("sph_unroll_mixed", 8)
+    ctx.set_param("sph_unroll_mixed", 4)
     ctx.set_param("sph_balance", 0)
 
 

@@ -67,7 +67,7 @@ struct tm_ctx {
   int64_t nbr_cap = 48;
   int sph_block = 128;
   int sph_unroll_fp64 = 4;   // neighbours fetched per trip in the FP64 pair kernel
-  int sph_unroll_mixed = 8;  // the same for the mixed kernel
+  int sph_unroll_mixed = 4;  // the same for the mixed kernel
   int nbr_block = 256;
   int sph_lanes = 1;
   int sph_balance = 0;  // FP64 kernel without tiles: rank cells by list length inside each workgroup

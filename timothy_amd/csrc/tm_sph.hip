@@ -318,11 +318,29 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
   double2 *const lzs = lds + lds_cap;
   const int64_t g = tm_xcd_block();
   const uint32_t t = threadIdx.x;
+  // The tile's size, the first staging indices and the rank map are read
+  // together, before the size is known (all lie inside their arrays; values
+  // of an over-full tile are never used), so staging waits on one load
+  // round trip for the indices instead of two.
+  const uint32_t *ti = tile_idx + g * TM_TILE_MAX;
+  constexpr int SPEC = 3;  // 3 TM_TILE = 768 <= TM_TILE_MAX indices ahead
+  uint32_t si[SPEC];
+#pragma unroll
+  for (int j = 0; j < SPEC; ++j) si[j] = ti[t + TM_TILE * j];
+  const uint32_t rank_cell = tile_cell[g * TM_TILE + t];
   const uint32_t m = __builtin_amdgcn_readfirstlane(tile_cnt[g]);
   const bool staged = m < lds_cap;  // m records and the stand-in fit (TM_TILE_OVER never does)
   if (staged) {
-    const uint32_t *ti = tile_idx + g * TM_TILE_MAX;
-    for (uint32_t i = t; i < m; i += TM_TILE) {
+#pragma unroll
+    for (int j = 0; j < SPEC; ++j) {
+      const uint32_t i = t + TM_TILE * j;
+      if (i < m) {
+        const double4 q = pos[si[j]];
+        lxy[i] = make_double2(q.x, q.y);
+        lzs[i] = make_double2(q.z, q.w);
+      }
+    }
+    for (uint32_t i = t + TM_TILE * SPEC; i < m; i += TM_TILE) {
       const double4 q = pos[ti[i]];
       lxy[i] = make_double2(q.x, q.y);
       lzs[i] = make_double2(q.z, q.w);
@@ -332,7 +350,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
       lzs[m] = make_double2(TM_FAR, 0.0);
     }
   }
-  const uint32_t off = staged ? (uint32_t)tile_cell[g * TM_TILE + t] : t;
+  const uint32_t off = staged ? (uint32_t)rank_cell : t;
   const int64_t cell = g * TM_TILE + off;
   const bool live = cell < n;
   const int64_t p = live ? cell : n - 1;
