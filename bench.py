@@ -8,8 +8,10 @@ forces over the neighbour lists), mixed precision and plain FP64, each
 bracketed by HIP events on the library's stream and synchronised on both
 sides.  `value` is the mixed form's pair-interactions/s (P / t_sph, P =
 ordered in-cutoff pairs); the FP64 drop-in, the full-step time (sort + tree +
-neighbour lists + SPH + CIC deposit to a 256^3 FP64 grid) and the
-mixed-vs-FP64 errors are reported beside it.
+neighbour lists + SPH + CIC deposit to a 256^3 FP64 grid), the resident
+mechanics loop (a captured graph whose every replay sorts, builds, lists,
+sums and moves the cells) and the mixed-vs-FP64 errors are reported beside
+it.
 
 --workload batched: 256 independent colonies of 1,000..200,000 cells
 (log-uniform, ~9.6M cells), one launch per stage over all colonies, the step
@@ -49,8 +51,8 @@ METRIC = "SPH density+force pair-interactions/s, mixed vs FP64 plain, 1 MI355X"
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", choices=["single", "batched", "large"], default="single")
     ap.add_argument("--scale", type=float, default=1.0, help="cell-count scale (1.0 = the named workload)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -259,6 +261,21 @@ def run_single(a, tm, gen):
     cic_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
     prod.free()
     copy_gbs = copy_bandwidth(tm, ctx)
+    # resident mechanics loop: every replay of the captured graph is one whole
+    # step of the colony in HBM -- sort, tree, lists, mixed SPH, and the move
+    # (updatepositions with counter-based draws); wall time per replay,
+    # including the host's check of the overflow flags
+    ctx.graph_capture_loop(tm.Move.make([0.05], seed=1, step=0), tm.MIXED, None)
+    for _ in range(2):
+        ctx.graph_launch()
+    ctx.sync()
+    t0 = time.perf_counter()
+    nloop = max(3, a.steps // 20)
+    for _ in range(nloop):
+        ctx.graph_launch()
+    ctx.sync()
+    loop_ms = (time.perf_counter() - t0) * 1e3 / nloop
+    ctx.graph_destroy()
 
     cpu = None
     if not a.no_cpu:
@@ -278,6 +295,7 @@ def run_single(a, tm, gen):
         "interp_fp32_vs_fp64_error_256^3": ierr,
         "full_step_ms": {"total": step_ms + cic_ms, "build": build_ms, "neighbors": nbr_ms,
                          "sph_mixed": out["mixed"]["ms"] if "mixed" in out else None, "cic_deposit_256^3": cic_ms},
+        "mechanics_loop_ms_per_step": loop_ms,
         "copy_kernel_gbs": copy_gbs,
         "roofline": roofline(B, mixed["ms"], traffic_from_profiles()),
         "cpu_baseline": cpu,

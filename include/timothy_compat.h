@@ -43,6 +43,13 @@ typedef int tm_mpi_request_t;
 #define ZOLTAN_ID_TYPE int
 #endif
 
+/* src/global.h:59-61: edges of the computational box */
+#ifndef BOXSIZEX
+#define BOXSIZEX 2048
+#define BOXSIZEY 2048
+#define BOXSIZEZ 2048
+#endif
+
 /* src/global.h:90-98 */
 typedef struct cellcount_t {
   uint64_t n, g0phase, g1phase, sphase, g2phase, mphase, necroticphase;
@@ -202,6 +209,11 @@ void patches_cells2envwait(systeminfo_t systeminfo, settings_t settings, patches
                            environment_t **environment);
 void patches_cells2envinit(systeminfo_t systeminfo, settings_t settings, patches_t *patches,
                            cellsinfo_t *cellsinfo, grid_t *grid);
+
+/* ---- src/cells.c:342-365 (defined in cells.c, not declared in cells.h:
+ * remove it from cells.c when linking this library) ---- */
+void updatepositions(settings_t settings, celltype_t *celltype, cellsinfo_t *cellsinfo, unsigned char *removecell,
+                     int64_t *removecount);
 
 /* ---- src/fieldgradient.h ---- */
 void fieldgradient(systeminfo_t systeminfo, settings_t settings, environment_t **environment, grid_t *grid);

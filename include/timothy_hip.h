@@ -107,7 +107,10 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * "sph_unroll_mixed" neighbours fetched per trip of the pair kernels (1, 2, 4,
  * 8), "sph_lanes" lanes per cell of the mixed kernel (1 = one lane per cell),
  * "sph_balance" (1 = the FP64 kernel ranks the cells of a workgroup by list
- * length); read-only "nbr_alloc" (slots per cell of the current lists) */
+ * length), "sph_tiles" (1 = tm_neighbors also builds LDS tiles and the 3-D
+ * pair kernels read neighbours from them; default 1); read-only "nbr_alloc"
+ * (slots per cell of the current lists) and
+ * "tile_lds" (records staged per tile, 0 without tiles) */
 tm_status tm_set_param(tm_ctx *ctx, const char *key, int64_t value);
 tm_status tm_get_param(tm_ctx *ctx, const char *key, int64_t *value);
 
@@ -189,6 +192,41 @@ tm_status tm_memset(tm_ctx *ctx, void *dst, int value, size_t bytes);
  * reference transfer used to calibrate memory-traffic counters) */
 tm_status tm_memcpy_dtod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 
+/* -------- moving the cells -------- */
+/* updatepositions (src/cells.c:342-365) on the uploaded cells, in HBM: each
+ * cell moves by the force of the last tm_sph, then per axis (z only in 3-D)
+ * by randommove[ctype] * (u * 2.0 - 1.0), in the reference's order and
+ * FP64 arithmetic; a cell whose sphere reaches a face of the box
+ * [-box/2, box/2] (outsidethebox, src/cells.c:41-58) gets removecell = 1.
+ * u: draws, when not NULL, is a DEVICE array of dimension*ncells floats in
+ * the reference's order (cell-major; x, y, then z), e.g. (float)rand()/RAND_MAX;
+ * otherwise u is word a of Philox4x32-10 with counter (cell, step) and key
+ * seed, as (float)(w >> 1) / 2147483648.0f.
+ * removecell: DEVICE array of ncells bytes (reference order) or NULL; cells
+ * already marked are not counted again.  *removecount (if not NULL) is
+ * increased by the number of cells newly marked (with NULL removecell: by the
+ * number outside).  Needs cells from tm_upload (bound cells are read-only)
+ * and tm_sph results; afterwards the tree is stale until the next tm_build. */
+typedef struct tm_move {
+  float randommove[16]; /* per cell type, celltype_t.randommove (src/global.h:334) */
+  double box[3];        /* box edges; the reference's is 2048^3 (BOXSIZEX/Y/Z, src/global.h:59-61) */
+  uint64_t seed;        /* Philox key */
+  uint64_t step;        /* Philox counter, high half (one value per step) */
+} tm_move;
+tm_status tm_update_positions(tm_ctx *ctx, const tm_move *move, const float *draws, uint8_t *removecell,
+                              int64_t *removecount);
+/* The same update on plain DEVICE arrays in cell order (no colony needed):
+ * x, y, z moved in place by the forces fx, fy, fz; size and ctype (0..15)
+ * per cell; randommove indexed by ctype.  Synchronises; *removecount as
+ * above. */
+tm_status tm_move_cells(tm_ctx *ctx, int64_t ncells, int32_t dimension, double *x, double *y, double *z,
+                        const double *size, const int32_t *ctype, const double *fx, const double *fy,
+                        const double *fz, const tm_move *move, const float *draws, uint8_t *removecell,
+                        int64_t *removecount);
+/* current positions of the cells, reference order, into host arrays (any may
+ * be NULL) */
+tm_status tm_get_positions(tm_ctx *ctx, double *x, double *y, double *z);
+
 /* -------- measurement -------- */
 /* runs `warmup` untimed then `iters` timed launches of a stage on the
  * context's stream, bracketed by HIP events on that stream; ms_per_iter is
@@ -204,6 +242,15 @@ tm_status tm_last_kernel_ms(tm_ctx *ctx, double *ms);
 /* capture build + neighbors + sph into a hipGraph (no allocation inside);
  * tm_graph_launch replays it and checks the overflow / degeneracy flags */
 tm_status tm_graph_capture(tm_ctx *ctx, tm_precision precision);
+/* the same step followed by tm_update_positions with Philox draws: each
+ * replay of the graph is one mechanics step of the resident colony (sort,
+ * tree, lists, sums, move), the first with move->step, then step + 1, ...
+ * A replay whose neighbour lists overflow moves nothing on the device;
+ * tm_graph_launch then grows the lists, redoes that step eagerly and
+ * recaptures. */
+tm_status tm_graph_capture_loop(tm_ctx *ctx, tm_precision precision, const tm_move *move, uint8_t *removecell);
+/* cells newly marked by captured-loop replays since the last call */
+tm_status tm_graph_removed(tm_ctx *ctx, int64_t *count);
 tm_status tm_graph_launch(tm_ctx *ctx);
 tm_status tm_graph_destroy(tm_ctx *ctx);
 

@@ -571,6 +571,75 @@ void or_fieldgradient(const int64_t ls[3], float resolution, const double *f, do
 #undef NODE
 }
 
+/* Philox4x32-10: one round multiplies counter words 0 and 2 by the two
+ * Philox multipliers and mixes the 64-bit products' halves with words 1, 3
+ * and the key; the key advances by the Weyl constants between rounds. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; r++) {
+    if (r > 0) {
+      k0 += W0;
+      k1 += W1;
+    }
+    const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+void or_philox_draws(int64_t n, int dimension, uint64_t seed, uint64_t step, float *draws) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int64_t c = 0; c < n; c++) {
+    const uint32_t ctr[4] = {(uint32_t)c, (uint32_t)((uint64_t)c >> 32), (uint32_t)step, (uint32_t)(step >> 32)};
+    uint32_t w[4];
+    or_philox4x32_10(ctr, key, w);
+    for (int a = 0; a < dimension; a++) draws[dimension * c + a] = (float)(w[a] >> 1) / 2147483648.0f;
+  }
+}
+
+/* src/cells.c:41-58 */
+static int outsidethebox(int dimension, double x, double y, double z, double r, const double box[3]) {
+  if (x - r <= -box[0] / 2.0 || x + r >= box[0] / 2.0) return 1;
+  if (y - r <= -box[1] / 2.0 || y + r >= box[1] / 2.0) return 1;
+  if (dimension == 3 && (z - r <= -box[2] / 2.0 || z + r >= box[2] / 2.0)) return 1;
+  return 0;
+}
+
+/* src/cells.c:342-365 */
+int64_t or_updatepositions(int64_t n, int dimension, double *x, double *y, double *z, const double *size,
+                           const int32_t *ctype, const float *randommove, const double *fx, const double *fy,
+                           const double *fz, const float *draws, const double box[3], uint8_t *removecell) {
+  int64_t removed = 0, k = 0;
+  for (int64_t c = 0; c < n; c++) {
+    const float rm = randommove[ctype[c]];
+    x[c] += fx[c];
+    y[c] += fy[c];
+    if (dimension == 3) z[c] += fz[c];
+    const float ux = draws ? draws[k++] : (float)rand() / RAND_MAX;
+    x[c] += rm * (ux * 2.0 - 1.0);
+    const float uy = draws ? draws[k++] : (float)rand() / RAND_MAX;
+    y[c] += rm * (uy * 2.0 - 1.0);
+    if (dimension == 3) {
+      const float uz = draws ? draws[k++] : (float)rand() / RAND_MAX;
+      z[c] += rm * (uz * 2.0 - 1.0);
+    }
+    if (outsidethebox(dimension, x[c], y[c], z[c], size[c], box) && removecell[c] == 0) {
+      removecell[c] = 1;
+      removed++;
+    }
+  }
+  return removed;
+}
+
 double or_wtime(void) {
 #ifdef _OPENMP
   return omp_get_wtime();

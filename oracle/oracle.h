@@ -27,6 +27,12 @@
  *                        partition of unity, mass conservation, exact
  *                        interpolation of trilinear fields, hand-computed indices
  *   or_fieldgradient     linear / quadratic fields (closed form)
+ *   or_philox4x32_10     the known-answer vectors published with the
+ *                        generator (Random123's kat_vectors)
+ *   or_updatepositions   hand-computed moves for chosen draws (u = 0, 1/2,
+ *                        1), glibc's documented rand() sequence for srand(1),
+ *                        cells placed exactly on the box faces, draw order
+ *                        in 2-D and 3-D
  */
 #ifndef TM_ORACLE_H
 #define TM_ORACLE_H
@@ -129,6 +135,28 @@ void or_cic_interp(const or_cells *c, const double lower[3], float resolution,
                    const double *field, const double *gradient, double *out);
 /* src/fieldgradient.c:149-224, no neighbour ranks (all faces one-sided) */
 void or_fieldgradient(const int64_t localsize[3], float resolution, const double *field, double *gradient);
+
+/* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ten rounds over
+ * a 4-word counter with a 2-word key bumped by the Weyl constants between
+ * rounds. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* the draws of the device path's counter-based moves: cell c, axis a takes
+ * word a of philox(ctr = (c lo, c hi, step lo, step hi), key = (seed lo,
+ * seed hi)) as (float)(w >> 1) / 2147483648.0f, the reference's
+ * (float)rand()/RAND_MAX applied to a 31-bit value.  draws is dim*n,
+ * cell-major. */
+void or_philox_draws(int64_t n, int dimension, uint64_t seed, uint64_t step, float *draws);
+/* src/cells.c:342-365 (updatepositions) with outsidethebox, src/cells.c:41-58:
+ * x += F, then x += randommove[type] * (u * 2.0 - 1.0) per axis (z only in
+ * 3-D), u in the reference's order (cell-major; x, y, then z); cells whose
+ * sphere reaches a face of the box [-box/2, box/2] get removecell = 1.
+ * draws == NULL takes u from the C library's rand() like the reference,
+ * otherwise dim*n values.  box: edge lengths (the reference's BOXSIZEX/Y/Z,
+ * src/global.h:59-61).  Returns the number of cells newly marked. */
+int64_t or_updatepositions(int64_t n, int dimension, double *x, double *y, double *z, const double *size,
+                           const int32_t *ctype, const float *randommove, const double *fx, const double *fy,
+                           const double *fz, const float *draws, const double box[3], uint8_t *removecell);
 
 /* omp_get_wtime wrapper and thread control for the timed baseline */
 double or_wtime(void);

@@ -20,6 +20,7 @@ i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
@@ -92,6 +93,13 @@ def lib():
         L.or_wtime.restype = C.c_double
         L.or_max_threads.restype = C.c_int
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.or_philox4x32_10.restype = None
+        L.or_philox_draws.argtypes = [C.c_int64, C.c_int, C.c_uint64, C.c_uint64, f32p]
+        L.or_philox_draws.restype = None
+        L.or_updatepositions.argtypes = [C.c_int64, C.c_int, f64p, f64p, f64p, f64p, i32p, f32p, f64p, f64p, f64p,
+                                         C.c_void_p, f64p, C.c_void_p]
+        L.or_updatepositions.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -270,6 +278,36 @@ def fieldgradient(localsize, resolution, field):
     g = np.zeros(int(ls[0] * ls[1] * ls[2]) * 3, np.float64)
     lib().or_fieldgradient(ls, np.float32(resolution), np.ascontiguousarray(field, np.float64), g)
     return g
+
+
+def philox4x32_10(ctr, key):
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(np.ascontiguousarray(ctr, np.uint32), np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+def philox_draws(n, dimension, seed, step):
+    d = np.zeros(max(1, n * dimension), np.float32)
+    lib().or_philox_draws(n, dimension, seed, step, d)
+    return d[: n * dimension]
+
+
+def updatepositions(x, y, z, size, ctype, randommove, fx, fy, fz, draws=None, box=(2048.0, 2048.0, 2048.0),
+                    removecell=None, dimension=3):
+    """src/cells.c:342-365 on copies of x, y, z (and removecell); draws None
+    takes them from the C library's rand() like the reference.  Returns
+    (x, y, z, removecell, newly_removed)."""
+    x, y, z = (np.array(a, np.float64) for a in (x, y, z))
+    n = len(x)
+    rc = np.zeros(n, np.uint8) if removecell is None else np.array(removecell, np.uint8)
+    dr = None if draws is None else np.ascontiguousarray(draws, np.float32)
+    k = lib().or_updatepositions(n, dimension, x, y, z, np.ascontiguousarray(size, np.float64),
+                                 np.ascontiguousarray(ctype, np.int32), np.ascontiguousarray(randommove, np.float32),
+                                 np.ascontiguousarray(fx, np.float64), np.ascontiguousarray(fy, np.float64),
+                                 np.ascontiguousarray(fz, np.float64),
+                                 None if dr is None else dr.ctypes.data_as(C.c_void_p),
+                                 np.ascontiguousarray(box, np.float64), rc.ctypes.data_as(C.c_void_p))
+    return x, y, z, rc, int(k)
 
 
 def wtime():

@@ -37,8 +37,12 @@ static int intersects(const octnode_t *o, uint3dv_t mn, uint3dv_t mx) {
   return 1;
 }
 
+/* usage: step_harness IN OUT [SEED RANDOMMOVE MOVE_OUT]: with the last three,
+ * cellsupdate's updatepositions (src/cells.c:538) follows the step after
+ * srand(SEED), every cell type taking RANDOMMOVE, and MOVE_OUT receives x, y,
+ * z, removecell, removecount and the next rand() value */
 int main(int argc, char **argv) {
-  if (argc != 3) return 1;
+  if (argc != 3 && argc != 6) return 1;
   FILE *f = fopen(argv[1], "rb");
   if (!f) return 1;
   int64_t hdr[5];
@@ -183,6 +187,24 @@ int main(int argc, char **argv) {
   fwrite(woff, 8, nwalk + 1, o);
   fwrite(wc, 4, wn, o);
   fclose(o);
+
+  if (argc == 6) {
+    srand((unsigned)atoi(argv[3]));
+    for (int k = 0; k < ntypes; k++) ct[k].randommove = (float)atof(argv[4]);
+    unsigned char *removecell = calloc(n, 1);
+    int64_t removecount = 0;
+    updatepositions(st, ct, &ci, removecell, &removecount);
+    const int64_t next = rand();
+    FILE *m = fopen(argv[5], "wb");
+    for (int64_t i = 0; i < n; i++) fwrite(&ci.cells[i].x, 8, 1, m);
+    for (int64_t i = 0; i < n; i++) fwrite(&ci.cells[i].y, 8, 1, m);
+    for (int64_t i = 0; i < n; i++) fwrite(&ci.cells[i].z, 8, 1, m);
+    fwrite(removecell, 1, n, m);
+    fwrite(&removecount, 8, 1, m);
+    fwrite(&next, 8, 1, m);
+    fclose(m);
+    free(removecell);
+  }
 
   octfree(&ci);
   for (int64_t k = 0; k < nf; k++) free(ced[k]);

@@ -65,7 +65,7 @@ def _read_output(path, n, G, nf, nwalk):
                 woff=woff, wc=wc)
 
 
-def _run(tmp_path, col, precision):
+def _run(tmp_path, col, precision, move=None):
     if not os.path.exists(B.HARNESS):
         pytest.fail(f"{B.HARNESS} is not built (run the package build first)")
     grid = col["grid"]
@@ -77,9 +77,20 @@ def _run(tmp_path, col, precision):
     _write_input(inp, col, fields, grads)
     env = dict(os.environ, TM_PRECISION=precision)
     print(f"step_harness n={col['n']} G={G} precision={precision}", flush=True)
-    r = subprocess.run([B.HARNESS, inp, out], env=env, capture_output=True, text=True, timeout=120)
+    args = [B.HARNESS, inp, out]
+    if move is not None:
+        args += [str(move[0]), repr(float(move[1])), str(tmp_path / "move.bin")]
+    r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    return _read_output(out, col["n"], G, len(fields), min(NWALK, col["n"])), fields, grads
+    got = _read_output(out, col["n"], G, len(fields), min(NWALK, col["n"]))
+    if move is not None:
+        b = open(tmp_path / "move.bin", "rb").read()
+        n = col["n"]
+        xyz = np.frombuffer(b, np.float64, 3 * n).reshape(3, n)
+        rc = np.frombuffer(b, np.uint8, n, 24 * n)
+        tail = np.frombuffer(b, np.int64, 2, 25 * n)
+        got.update(x=xyz[0], y=xyz[1], z=xyz[2], removecell=rc, removecount=int(tail[0]), next_rand=int(tail[1]))
+    return got, fields, grads
 
 
 CASES = [
@@ -138,3 +149,37 @@ def test_step_compute_order_mixed(oracle_mod, tmp_path):
     assert np.array_equal(c[:, 2], md)
     assert np.abs(c[:, 0] - dens).max() <= 1e-5 * np.abs(dens).max()
     assert np.abs(c[:, 3:6] - F).max() <= 1e-5 * np.abs(F).max()
+
+
+def test_updatepositions_after_step(oracle_mod, tmp_path):
+    """cellsupdate's updatepositions (src/cells.c:342-365) through the
+    reference API after the step: the random steps come from the program's
+    rand() in the reference's order (the stream advances by 3 draws per
+    cell), positions are bit-exact with the oracle fed the same rand()
+    stream, and cells reaching the box faces are marked and counted"""
+    import ctypes
+
+    O = oracle_mod
+    col = gen.config(0)
+    extra = np.array([[1023.2, 0.0, 0.0], [0.0, -1023.4, 0.0], [0.0, 0.0, 1022.5]])
+    for k, a in enumerate("xyz"):
+        col[a] = np.concatenate([col[a], extra[:, k]])
+    m = len(extra)
+    col["size"] = np.concatenate([col["size"], np.ones(m)])
+    col["young"] = np.concatenate([col["young"], np.full(m, 2100.0, np.float32)])
+    col["ctype"] = np.concatenate([col["ctype"], np.zeros(m, np.int32)])
+    col["phase"] = np.concatenate([col["phase"], np.ones(m, np.int32)])
+    col["n"] = len(col["x"])
+    col["grid"] = gen.bbox_grid(col, 64)
+    got, _, _ = _run(tmp_path, col, "fp64", move=(2024, 0.3))
+    F = got["cells"][:, 3:6]
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    assert np.array_equal(F, tree.computegradient())
+    libc = ctypes.CDLL(None)
+    libc.rand.restype = ctypes.c_int
+    libc.srand(2024)
+    ex, ey, ez, erc, ek = O.updatepositions(col["x"], col["y"], col["z"], col["size"], col["ctype"], [0.3],
+                                            F[:, 0], F[:, 1], F[:, 2], None)
+    assert np.array_equal(got["x"], ex) and np.array_equal(got["y"], ey) and np.array_equal(got["z"], ez)
+    assert np.array_equal(got["removecell"], erc) and got["removecount"] == ek == 2
+    assert got["next_rand"] == libc.rand()

@@ -277,3 +277,103 @@ def test_fieldgradient_linear_and_quadratic(oracle_mod):
     q = (x * x).reshape(-1)
     gq = O.fieldgradient(ls, res, q).reshape(ls + (3,))
     assert np.allclose(gq[1:-1, :, :, 0], 2 * (2 * x[1:-1]))
+
+
+# ---- moving the cells (src/cells.c:342-365) and the counter-based draws ----
+
+PHILOX_KAT = [  # Random123 kat_vectors, philox4x32 10 rounds: counter, key, result
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+def test_philox_known_answers(oracle_mod):
+    for ctr, key, want in PHILOX_KAT:
+        assert list(oracle_mod.philox4x32_10(ctr, key)) == want
+
+
+def test_philox_draws_layout(oracle_mod):
+    """cell c, step s, seed k: counter (c lo, c hi, s lo, s hi), key (k lo, k hi),
+    words x, y, z scaled as 31-bit values; cell 0 / step 0 / seed 0 is the
+    first known-answer vector"""
+    d = oracle_mod.philox_draws(1, 3, 0, 0)
+    want = [np.float32(w >> 1) / np.float32(2.0 ** 31) for w in PHILOX_KAT[0][2][:3]]
+    assert np.array_equal(d, np.array(want, np.float32))
+    seed, step = (0x299F31D0 << 32) | 0xA4093822, (7 << 32) | 5
+    d = oracle_mod.philox_draws(4, 2, seed, step).reshape(4, 2)
+    for c in range(4):
+        w = oracle_mod.philox4x32_10([c, 0, 5, 7], [0xA4093822, 0x299F31D0])
+        assert np.array_equal(d[c], (w[:2] >> 1).astype(np.float32) / np.float32(2.0 ** 31))
+    assert d.min() >= 0.0 and d.max() <= 1.0
+
+
+def _move_case():
+    # two cells, types 0 and 1; forces and draws chosen so every result is exact
+    x, y, z = np.array([0.0, 10.0]), np.array([1.0, -3.0]), np.array([2.0, 4.0])
+    size = np.array([1.0, 1.5])
+    ctype = np.array([0, 1], np.int32)
+    rm = np.array([0.5, 0.25], np.float32)
+    fx, fy, fz = np.array([0.25, -1.0]), np.array([0.0, 0.5]), np.array([-0.5, 0.125])
+    return x, y, z, size, ctype, rm, fx, fy, fz
+
+
+def test_updatepositions_hand_computed(oracle_mod):
+    O = oracle_mod
+    x, y, z, size, ctype, rm, fx, fy, fz = _move_case()
+    # u = 1 -> +randommove, u = 0 -> -randommove, u = 1/2 -> no random step
+    draws = np.array([1.0, 0.0, 0.5, 0.5, 1.0, 0.0], np.float32)
+    nx, ny, nz, rc, k = O.updatepositions(x, y, z, size, ctype, rm, fx, fy, fz, draws)
+    assert list(nx) == [0.75, 9.0] and list(ny) == [0.5, -2.25] and list(nz) == [1.5, 3.875]
+    assert k == 0 and not rc.any()
+    # 2-D: two draws per cell, z untouched even with a z force
+    draws2 = np.array([1.0, 0.0, 0.0, 1.0], np.float32)
+    nx, ny, nz, rc, k = O.updatepositions(x, y, z, size, ctype, rm, fx, fy, fz, draws2, dimension=2)
+    assert list(nx) == [0.75, 8.75] and list(ny) == [0.5, -2.25] and list(nz) == [2.0, 4.0]
+
+
+def test_updatepositions_box_faces(oracle_mod):
+    """outsidethebox (src/cells.c:41-58): a sphere touching a face counts as
+    outside (<=, >=); marked cells are not counted again; 2-D ignores z"""
+    O = oracle_mod
+    n = 6
+    x = np.array([-1023.0, 1023.0, 1022.999, 0.0, 0.0, -1023.0])
+    y = np.array([0.0, 0.0, 0.0, 1023.0, 0.0, 0.0])
+    z = np.array([0.0, 0.0, 0.0, 0.0, -1023.5, 0.0])
+    size = np.ones(n)
+    ctype = np.zeros(n, np.int32)
+    zero = np.zeros(n)
+    half = np.full(3 * n, 0.5, np.float32)
+    pre = np.array([0, 0, 0, 0, 0, 1], np.uint8)
+    nx, ny, nz, rc, k = O.updatepositions(x, y, z, size, ctype, [0.3], zero, zero, zero, half, removecell=pre)
+    assert list(rc) == [1, 1, 0, 1, 1, 1] and k == 4
+    nx, ny, nz, rc, k = O.updatepositions(x, y, z, size, ctype, [0.3], zero, zero, zero, half[: 2 * n],
+                                          removecell=pre, dimension=2)
+    assert list(rc) == [1, 1, 0, 1, 0, 1] and k == 3
+    # a smaller box
+    nx, ny, nz, rc, k = O.updatepositions(x[2:3], y[2:3], z[2:3], size[:1], ctype[:1], [0.3], zero[:1], zero[:1],
+                                          zero[:1], half[:3], box=(4000.0, 4000.0, 4000.0))
+    assert k == 0
+
+
+def test_updatepositions_rand_stream_order(oracle_mod):
+    """with no draws the restatement takes rand() like the reference: after
+    srand(1) glibc yields its documented sequence, consumed cell after cell,
+    x then y then z"""
+    import ctypes
+
+    libc = ctypes.CDLL(None)
+    libc.rand.restype = ctypes.c_int
+    libc.srand(1)
+    seq = [libc.rand() for _ in range(6)]
+    assert seq == [1804289383, 846930886, 1681692777, 1714636915, 1957747793, 424238335]
+    O = oracle_mod
+    x, y, z, size, ctype, rm, fx, fy, fz = _move_case()
+    libc.srand(1)
+    a = O.updatepositions(x, y, z, size, ctype, rm, fx, fy, fz, None)
+    draws = (np.array(seq, np.float32) / np.float32(2147483647)).astype(np.float32)
+    b = O.updatepositions(x, y, z, size, ctype, rm, fx, fy, fz, draws)
+    for u, v in zip(a[:3], b[:3]):
+        assert np.array_equal(u, v)
+    assert libc.rand() == 719885386  # exactly six values were consumed

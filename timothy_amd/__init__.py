@@ -49,6 +49,22 @@ class Grid(C.Structure):
         return o
 
 
+class Move(C.Structure):
+    """tm_move: randommove per cell type, box edges, Philox key and step"""
+    _fields_ = [("randommove", C.c_float * 16), ("box", C.c_double * 3), ("seed", C.c_uint64),
+                ("step", C.c_uint64)]
+
+    @classmethod
+    def make(cls, randommove, box=(2048.0, 2048.0, 2048.0), seed=0, step=0):
+        o = cls()
+        rm = list(randommove) + [0.0] * (16 - len(randommove))
+        o.randommove[:] = [float(np.float32(v)) for v in rm]
+        o.box[:] = [float(v) for v in box]
+        o.seed = int(seed)
+        o.step = int(step)
+        return o
+
+
 _lib = None
 _vp = C.c_void_p
 _i64 = C.c_int64
@@ -104,6 +120,12 @@ def _load():
         "tm_graph_launch": (_i32, [_vp]),
         "tm_graph_destroy": (_i32, [_vp]),
         "tm_autotune": (_i32, [_vp, C.c_int, C.c_char_p, C.c_size_t]),
+        "tm_update_positions": (_i32, [_vp, C.POINTER(Move), _vp, _vp, C.POINTER(_i64)]),
+        "tm_move_cells": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Move), _vp, _vp,
+                                 C.POINTER(_i64)]),
+        "tm_get_positions": (_i32, [_vp, _vp, _vp, _vp]),
+        "tm_graph_capture_loop": (_i32, [_vp, C.c_int, C.POINTER(Move), _vp]),
+        "tm_graph_removed": (_i32, [_vp, C.POINTER(_i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -144,6 +166,11 @@ class DeviceArray:
         out = np.empty(self.shape, self.dtype)
         self.ctx._check(lib().tm_memcpy_dtoh(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
         return out
+
+    def set(self, arr):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        assert arr.nbytes == self.nbytes
+        self.ctx._check(lib().tm_memcpy_htod(self.ctx.h, self.ptr, arr.ctypes.data, self.nbytes))
 
     def zero(self):
         self.ctx._check(lib().tm_memset(self.ctx.h, self.ptr, 0, self.nbytes))
@@ -339,8 +366,41 @@ class Context:
     def graph_capture(self, precision=FP64):
         self._check(lib().tm_graph_capture(self.h, precision))
 
+    def graph_capture_loop(self, move: Move, precision=MIXED, removecell: "DeviceArray" = None):
+        self._check(lib().tm_graph_capture_loop(self.h, precision, C.byref(move),
+                                                None if removecell is None else removecell.ptr))
+
+    def graph_removed(self):
+        k = _i64()
+        self._check(lib().tm_graph_removed(self.h, C.byref(k)))
+        return k.value
+
+    # ---- moving the cells
+    def update_positions(self, move: Move, draws: "DeviceArray" = None, removecell: "DeviceArray" = None):
+        """tm_update_positions; returns the number of cells newly marked"""
+        k = _i64(0)
+        self._check(lib().tm_update_positions(self.h, C.byref(move), None if draws is None else draws.ptr,
+                                              None if removecell is None else removecell.ptr, C.byref(k)))
+        return k.value
+
+    def move_cells(self, n, dimension, x, y, z, size, ctype, fx, fy, fz, move: Move, draws=None, removecell=None):
+        """tm_move_cells on device arrays; returns the number of cells newly marked"""
+        k = _i64(0)
+        p = lambda a: None if a is None else a.ptr
+        self._check(lib().tm_move_cells(self.h, n, dimension, p(x), p(y), p(z), p(size), p(ctype), p(fx), p(fy),
+                                        p(fz), C.byref(move), p(draws), p(removecell), C.byref(k)))
+        return k.value
+
+    def positions(self):
+        out = [np.empty(self.n) for _ in range(3)]
+        self._check(lib().tm_get_positions(self.h, *[o.ctypes.data for o in out]))
+        return out
+
     def graph_launch(self):
         self._check(lib().tm_graph_launch(self.h))
+
+    def graph_destroy(self):
+        self._check(lib().tm_graph_destroy(self.h))
 
     def autotune(self, precision=MIXED):
         buf = C.create_string_buffer(4096)

@@ -56,6 +56,9 @@ static struct {
   double **d_envf, **d_envg;
   int env_nf, env_ready;
   int64_t env_nodes;
+  /* updatepositions staging (host and device, one block each) */
+  void *mv_host, *mv_dev;
+  int64_t mv_cap;
 } S;
 
 static void free_env(void);
@@ -130,8 +133,12 @@ void tm_compat_shutdown(void) {
     if (S.d_field) tm_free(S.ctx, S.d_field);
     if (S.d_grad) tm_free(S.ctx, S.d_grad);
     if (S.d_interp) tm_free(S.ctx, S.d_interp);
+    if (S.mv_dev) tm_free(S.ctx, S.mv_dev);
     tm_destroy(S.ctx);
   }
+  free(S.mv_host);
+  S.mv_host = S.mv_dev = NULL;
+  S.mv_cap = 0;
   S.d_prod = S.d_field = S.d_grad = S.d_interp = NULL;
   S.grid_cap = S.interp_cap = 0;
   S.ctx = NULL;
@@ -368,6 +375,84 @@ void computeremotegradient(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltyp
   FAIL(si, "halo cells (numimp > 0) need more than one rank; not served on one GPU");
   (void)ci;
   (void)celltype;
+}
+
+/* ------------------------------------------------------------------ */
+/* moving the cells                                                    */
+/* ------------------------------------------------------------------ */
+
+/* bytes per cell of the staging block: x, y, z, size, fx, fy, fz (56),
+ * ctype (4), three draws (12), removecell (1) */
+#define MV_BYTES 73
+
+/* src/cells.c:342-365.  The random steps are drawn from the program's rand()
+ * exactly as the reference draws them (cell after cell; x, y, then z), so the
+ * rest of the program sees the same rand() stream; the update itself runs on
+ * the GPU over the current cells and forces. */
+void updatepositions(settings_t st, celltype_t *celltype, cellsinfo_t *ci, unsigned char *removecell,
+                     int64_t *removecount) {
+  systeminfo_t si;
+  memset(&si, 0, sizeof(si));
+  si.size = 1;
+  const int64_t n = (int64_t)ci->localcount.n;
+  if (n <= 0) return;
+  const int dim = st.dimension == 2 ? 2 : 3;
+  tm_ctx *c = ctx_for(si);
+  if (st.numberofcelltypes < 1 || st.numberofcelltypes > 16) FAIL(si, "at most 16 cell types");
+  tm_move m;
+  memset(&m, 0, sizeof(m));
+  for (int t = 0; t < st.numberofcelltypes; t++) m.randommove[t] = celltype[t].randommove;
+  m.box[0] = BOXSIZEX;
+  m.box[1] = BOXSIZEY;
+  m.box[2] = BOXSIZEZ;
+  if (n > S.mv_cap) {
+    free(S.mv_host);
+    if (S.mv_dev) tm_free(c, S.mv_dev);
+    S.mv_dev = NULL;
+    S.mv_cap = 0;
+    S.mv_host = malloc((size_t)MV_BYTES * (size_t)n);
+    if (!S.mv_host) FAIL(si, "cannot allocate host staging");
+    check(si, tm_malloc(c, &S.mv_dev, (size_t)MV_BYTES * (size_t)n), "move staging");
+    S.mv_cap = n;
+  }
+  /* the same layout on host and device */
+  double *hx = (double *)S.mv_host, *hy = hx + n, *hz = hy + n, *hs = hz + n, *hf = hs + n;
+  int32_t *ht = (int32_t *)(hf + 3 * n);
+  float *hd = (float *)(ht + n);
+  uint8_t *hr = (uint8_t *)(hd + 3 * n);
+  double *dx = (double *)S.mv_dev, *dy = dx + n, *dz = dy + n, *ds = dz + n, *df = ds + n;
+  int32_t *dt = (int32_t *)(df + 3 * n);
+  float *dd = (float *)(dt + n);
+  uint8_t *dr = (uint8_t *)(dd + 3 * n);
+  int64_t k = 0;
+  for (int64_t i = 0; i < n; i++) {
+    const celldata_t *cd = &ci->cells[i];
+    hx[i] = cd->x;
+    hy[i] = cd->y;
+    hz[i] = cd->z;
+    hs[i] = cd->size;
+    hf[i] = ci->forces[i].x;
+    hf[n + i] = ci->forces[i].y;
+    hf[2 * n + i] = ci->forces[i].z;
+    if (cd->ctype < 0 || cd->ctype >= st.numberofcelltypes) FAIL(si, "ctype out of range");
+    ht[i] = cd->ctype;
+    for (int a = 0; a < dim; a++) hd[k++] = (float)rand() / RAND_MAX;
+    hr[i] = removecell[i];
+  }
+  check(si, tm_memcpy_htod(c, S.mv_dev, S.mv_host, (size_t)MV_BYTES * (size_t)n), "move upload");
+  int64_t added = 0;
+  check(si,
+        tm_move_cells(c, n, dim, dx, dy, dz, ds, dt, df, df + n, df + 2 * n, &m, dd, dr, &added),
+        "updatepositions");
+  check(si, tm_memcpy_dtoh(c, hx, dx, sizeof(double) * 3 * (size_t)n), "move download");
+  check(si, tm_memcpy_dtoh(c, hr, dr, (size_t)n), "move download");
+  for (int64_t i = 0; i < n; i++) {
+    ci->cells[i].x = hx[i];
+    ci->cells[i].y = hy[i];
+    if (dim == 3) ci->cells[i].z = hz[i];
+    removecell[i] = hr[i];
+  }
+  *removecount += added;
 }
 
 /* ------------------------------------------------------------------ */

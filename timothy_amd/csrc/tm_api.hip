@@ -698,16 +698,24 @@ tm_status tm_graph_destroy(tm_ctx *c) {
   return TM_OK;
 }
 
-tm_status tm_graph_capture(tm_ctx *c, tm_precision p) {
-  if (!c) return TM_ERR_INVALID;
-  if (c->stage < 1) return tm_fail(c, TM_ERR_STATE, "upload cells first");
-  // one eager step sizes every buffer, so the capture allocates nothing
-  TM_TRY(tm_step(c, p));
+namespace {
+
+tm_status check_move(tm_ctx *c, const tm_move *m) {
+  if (!m) return tm_fail(c, TM_ERR_INVALID, "move parameters missing");
+  if (c->external) return tm_fail(c, TM_ERR_STATE, "cells bound with tm_bind_device are read-only; use tm_upload");
+  if (!(m->box[0] > 0.0) || !(m->box[1] > 0.0) || !(m->box[2] > 0.0))
+    return tm_fail(c, TM_ERR_INVALID, "box edges must be positive");
+  return TM_OK;
+}
+
+// capture build + neighbors + sph (+ the move) on the context's stream
+tm_status capture(tm_ctx *c, tm_precision p) {
   tm_graph_destroy(c);
   TM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   tm_status s = tm_run_build(c);
   if (s == TM_OK) s = tm_run_neighbors(c, false);
   if (s == TM_OK) s = tm_run_sph(c, p);
+  if (s == TM_OK && c->graph_loop) s = tm_run_move(c, &c->graph_move, nullptr, c->graph_removecell, true);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (s != TM_OK) {
@@ -721,18 +729,102 @@ tm_status tm_graph_capture(tm_ctx *c, tm_precision p) {
   return TM_OK;
 }
 
+}  // namespace
+
+tm_status tm_update_positions(tm_ctx *c, const tm_move *m, const float *draws, uint8_t *removecell,
+                              int64_t *removecount) {
+  if (!c) return TM_ERR_INVALID;
+  TM_TRY(check_move(c, m));
+  if (c->stage < 4) return tm_fail(c, TM_ERR_STATE, "no forces: run tm_sph first");
+  if (c->moved) return tm_fail(c, TM_ERR_STATE, "the cells moved since the last sums: run tm_step first");
+  TM_TRY(tm_run_move(c, m, draws, removecell, false));
+  c->stage = 1;  // positions changed: the tree and the sums are stale
+  if (removecount) {
+    unsigned long long v = 0;
+    TM_HIP(c, hipMemcpyAsync(&v, tm_removed_word(c), sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    TM_HIP(c, hipStreamSynchronize(c->stream));
+    *removecount += (int64_t)v;
+  }
+  return TM_OK;
+}
+
+tm_status tm_move_cells(tm_ctx *c, int64_t n, int32_t dimension, double *x, double *y, double *z,
+                        const double *size, const int32_t *ctype, const double *fx, const double *fy,
+                        const double *fz, const tm_move *m, const float *draws, uint8_t *removecell,
+                        int64_t *removecount) {
+  if (!c) return TM_ERR_INVALID;
+  if (!m) return tm_fail(c, TM_ERR_INVALID, "move parameters missing");
+  if (n < 0 || (dimension != 2 && dimension != 3)) return tm_fail(c, TM_ERR_INVALID, "bad cell count or dimension");
+  if (n > 0 && (!x || !y || (dimension == 3 && (!z || !fz)) || !size || !ctype || !fx || !fy))
+    return tm_fail(c, TM_ERR_INVALID, "missing cell or force array");
+  if (!(m->box[0] > 0.0) || !(m->box[1] > 0.0) || !(m->box[2] > 0.0))
+    return tm_fail(c, TM_ERR_INVALID, "box edges must be positive");
+  TM_TRY(tm_run_move_arrays(c, n, dimension, x, y, z, size, ctype, fx, fy, fz, m, draws, removecell));
+  unsigned long long w[2] = {0, 0};  // newly marked, error bits
+  TM_HIP(c, hipMemcpyAsync(&w[0], tm_removed_word(c), 8, hipMemcpyDeviceToHost, c->stream));
+  TM_HIP(c, hipMemcpyAsync(&w[1], tm_move_error_word(c), 8, hipMemcpyDeviceToHost, c->stream));
+  TM_HIP(c, hipStreamSynchronize(c->stream));
+  if (w[1] & TM_FLAG_CTYPE) return tm_fail(c, TM_ERR_INVALID, "ctype out of range (0..15)");
+  if (removecount) *removecount += (int64_t)w[0];
+  return TM_OK;
+}
+
+tm_status tm_get_positions(tm_ctx *c, double *x, double *y, double *z) {
+  if (!c) return TM_ERR_INVALID;
+  if (c->stage < 1) return tm_fail(c, TM_ERR_STATE, "no cells");
+  const int64_t n = c->n;
+  if (n == 0) return TM_OK;
+  if (x) TM_HIP(c, hipMemcpyAsync(x, c->x, 8 * n, hipMemcpyDeviceToHost, c->stream));
+  if (y) TM_HIP(c, hipMemcpyAsync(y, c->y, 8 * n, hipMemcpyDeviceToHost, c->stream));
+  if (z) TM_HIP(c, hipMemcpyAsync(z, c->z, 8 * n, hipMemcpyDeviceToHost, c->stream));
+  TM_HIP(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+tm_status tm_graph_capture_loop(tm_ctx *c, tm_precision p, const tm_move *m, uint8_t *removecell) {
+  if (!c) return TM_ERR_INVALID;
+  if (c->stage < 1) return tm_fail(c, TM_ERR_STATE, "upload cells first");
+  TM_TRY(check_move(c, m));
+  // one eager step sizes every buffer, so the capture allocates nothing
+  TM_TRY(tm_step(c, p));
+  c->graph_loop = true;
+  c->graph_move = *m;
+  c->graph_removecell = removecell;
+  TM_TRY(tm_set_graph_step(c, m->step));
+  tm_status s = capture(c, p);
+  if (s != TM_OK) c->graph_loop = false;
+  return s;
+}
+
+tm_status tm_graph_removed(tm_ctx *c, int64_t *count) {
+  if (!c) return TM_ERR_INVALID;
+  return tm_take_graph_removed(c, count);
+}
+
+tm_status tm_graph_capture(tm_ctx *c, tm_precision p) {
+  if (!c) return TM_ERR_INVALID;
+  if (c->stage < 1) return tm_fail(c, TM_ERR_STATE, "upload cells first");
+  // one eager step sizes every buffer, so the capture allocates nothing
+  TM_TRY(tm_step(c, p));
+  c->graph_loop = false;
+  return capture(c, p);
+}
+
 tm_status tm_graph_launch(tm_ctx *c) {
   if (!c) return TM_ERR_INVALID;
   if (!c->gexec) return tm_fail(c, TM_ERR_STATE, "no captured graph");
   TM_HIP(c, hipGraphLaunch(c->gexec, c->stream));
   TM_TRY(check_flags(c));
   if (c->h_flags[1] > (unsigned)c->nbr_alloc) {
-    // a cell outgrew the captured lists: grow, redo eagerly, recapture
+    // a cell outgrew the captured lists (and a captured move moved nothing):
+    // grow, redo the step eagerly from this replay's tree, recapture
     TM_TRY(tm_run_neighbors(c, true));
     TM_TRY(tm_run_sph(c, c->graph_prec));
-    return tm_graph_capture(c, c->graph_prec);
+    if (c->graph_loop) TM_TRY(tm_run_move(c, &c->graph_move, nullptr, c->graph_removecell, true));
+    TM_TRY(capture(c, c->graph_prec));
   }
   c->stage = 4;
+  c->moved = c->graph_loop;
   c->last_prec = c->graph_prec;
   return TM_OK;
 }

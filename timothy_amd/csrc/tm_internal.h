@@ -143,11 +143,15 @@ struct tm_ctx {
   // unpermuted outputs
   double *d_out = nullptr;
 
-  // flags: [0] error bits, [1] max neighbour count, [2] pairs low, [3] pairs high
+  // flags: [0] error bits, [1] max neighbour count, [2] pairs low, [3] pairs high,
+  // [4] largest tile, [6..7] cells newly marked by the last eager move (u64),
+  // [8..9] the same summed over captured-loop replays, [10..11] the captured
+  // loop's next Philox step, [12..13] error bits of the moves
   unsigned int *d_flags = nullptr;
   unsigned int *h_flags = nullptr;  // pinned
 
   int stage = 0;  // 0 empty, 1 cells, 2 built, 3 neighbours, 4 sph
+  bool moved = false;  // a captured loop moved the cells after its sums (cleared by the next build)
   tm_precision last_prec = TM_FP64;
 
   // last deposit, for timing
@@ -163,6 +167,9 @@ struct tm_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   tm_precision graph_prec = TM_FP64;
+  bool graph_loop = false;     // the captured step also moves the cells
+  tm_move graph_move{};        // its parameters
+  uint8_t *graph_removecell = nullptr;
 };
 
 // error helpers
@@ -204,6 +211,19 @@ tm_status tm_run_sph(tm_ctx *c, tm_precision prec);
 tm_status tm_run_unpermute(tm_ctx *c);
 tm_status tm_run_deposit(tm_ctx *c, const tm_grid *g, int weight_mode, double *prod);
 tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count);
+// moves the cells by the forces of the last tm_run_sph; graph: Philox step from
+// the device word, advanced afterwards, count summed into the loop's word, and
+// nothing moves when the neighbour lists of this step overflowed
+tm_status tm_run_move(tm_ctx *c, const tm_move *m, const float *draws, uint8_t *removecell, bool graph);
+tm_status tm_set_graph_step(tm_ctx *c, uint64_t step);
+tm_status tm_take_graph_removed(tm_ctx *c, int64_t *count);
+inline unsigned long long *tm_removed_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 6); }
+inline unsigned long long *tm_graph_removed_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 8); }
+inline unsigned long long *tm_graph_step_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 10); }
+inline unsigned long long *tm_move_error_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 12); }
+tm_status tm_run_move_arrays(tm_ctx *c, int64_t n, int dim, double *x, double *y, double *z, const double *size,
+                             const int32_t *ctype, const double *fx, const double *fy, const double *fz,
+                             const tm_move *m, const float *draws, uint8_t *removecell);
 
 // small device helpers ------------------------------------------------------
 __device__ __forceinline__ unsigned long long tm_lanemask_lt() {
