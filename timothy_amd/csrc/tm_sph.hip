@@ -69,27 +69,41 @@ __device__ __forceinline__ void pair_fp64(const double4 &P, const double4 &Q, co
 // difference of squares, om = (h^2 - d^2) / ((h + d) h), so a near-cutoff
 // term keeps its relative accuracy.  Beyond the cutoff both vanish exactly
 // (om = m = 0), which is what makes the far stand-in record a zero term.
-struct MixedTerm {
-  float dw, coef, dx, dy, dz;
+// Per-cell constants of the mixed pairs: h (FP32), h^2 (FP32 and FP64), -2/h.
+struct MixedC {
+  float hf, hhf, m2hinv;
+  double hh;
 };
 
-__device__ __forceinline__ MixedTerm term_mixed(const double4 &P, const double4 &Q, float m2hinv, float hf,
-                                                float hhf, double hh) {
-  const double ex = Q.x - P.x, ey = Q.y - P.y, ez = Q.z - P.z;
+__device__ __forceinline__ MixedC mixed_consts(const TypeDev &T) {
+  const float hf = (float)T.h;
+  return MixedC{hf, hf * hf, -2.0f * T.hinv_f, T.h * T.h};
+}
+
+// One pair of the mixed form, added into the cell's sums: the separation
+// and its square in FP64, the spline in FP32 with d from one reciprocal
+// square root, om = 1 - d/h formed as (h^2 - d^2) / (h (h + d)) with h^2 - d^2
+// in FP64 (so om keeps its relative accuracy where it vanishes at the
+// cutoff, the terms there scaling as om^2 and om^3) and m = 1 - 2 d/h, both
+// clamped at 0 (W = 2 om^3 - m^3 covers both branches of the M4 kernel and
+// the far stand-in gives exact zeros); sw += size_q * W in FP32, and the
+// force coefficient (m^2 - om^2) / d times the FP64 separation added in FP64.
+template <bool DIM3>
+__device__ __forceinline__ void pair_mixed(const double4 &P, double qx, double qy, double qz, float sq,
+                                           const MixedC &K, float &sw, double &f0, double &f1, double &f2) {
+  const double ex = qx - P.x, ey = qy - P.y, ez = qz - P.z;
   const double d2d = __builtin_fma(ez, ez, __builtin_fma(ey, ey, ex * ex));
   const float d2 = (float)d2d;
   const float rinv = __builtin_amdgcn_rsqf(d2);
   const float d = d2 * rinv;
-  const float om = fmaxf((float)(hh - d2d), 0.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(d, hf, hhf));
-  const float m = fmaxf(__builtin_fmaf(d, m2hinv, 1.0f), 0.0f);
+  const float om = fmaxf((float)(K.hh - d2d), 0.0f) * __builtin_amdgcn_rcpf(__builtin_fmaf(d, K.hf, K.hhf));
+  const float m = fmaxf(__builtin_fmaf(d, K.m2hinv, 1.0f), 0.0f);
   const float om2 = om * om, m2 = m * m;
-  MixedTerm t;
-  t.dw = (float)Q.w * __builtin_fmaf(om2 + om2, om, -(m2 * m));
-  t.coef = (m2 - om2) * rinv;
-  t.dx = (float)ex;
-  t.dy = (float)ey;
-  t.dz = (float)ez;
-  return t;
+  sw = __builtin_fmaf(sq, __builtin_fmaf(om2 + om2, om, -(m2 * m)), sw);
+  const double coef = (double)((m2 - om2) * rinv);
+  f0 = __builtin_fma(coef, ex, f0);
+  f1 = __builtin_fma(coef, ey, f1);
+  if (DIM3) f2 = __builtin_fma(coef, ez, f2);
 }
 
 typedef unsigned int tm_u32x4 __attribute__((ext_vector_type(4)));
@@ -119,12 +133,12 @@ __device__ __forceinline__ double4 gather(const double4 *__restrict__ pos, __amd
 // uniform), fetch(q) the record.  Each trip fetches U records whose numbers
 // were read one trip earlier, then reads the next trip's numbers, then
 // evaluates the U pairs in list order.  The mixed form adds a trip's U terms
-// in FP32 and the trip's sums into the FP64 per-cell accumulators; its
-// padding slots hold the far stand-in record, whose terms are exactly zero.
+// of the kernel value in FP32 and the trip's sum into the FP64 per-cell
+// density, the force terms straight into FP64; its padding slots hold the
+// far stand-in record, whose terms are exactly zero.
 // The FP64 form evaluates only the slots below the cell's own count.
 template <bool MIXED, int U, int L, bool DIM3, class Slot, class Fetch>
-__device__ __forceinline__ void cell_sums(const double4 &P, const TypeFP64 &C64, float hf, float m2hinv, float hhf,
-                                          double hh, uint32_t cnt, uint32_t wmax, uint32_t far, Slot slot,
+__device__ __forceinline__ void cell_sums(const double4 &P, const TypeFP64 &C64, const MixedC &K, uint32_t cnt, uint32_t wmax, uint32_t far, Slot slot,
                                           Fetch fetch, double &de, double &f0, double &f1, double &f2) {
   constexpr uint32_t STEP = (uint32_t)(U * L);
   uint32_t nxt[U];
@@ -140,19 +154,11 @@ __device__ __forceinline__ void cell_sums(const double4 &P, const TypeFP64 &C64,
       nxt[j] = kb < wmax ? slot(kb) : far;
     }
     if constexpr (MIXED) {
-      float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+      float sw = 0.0f;
 #pragma unroll
-      for (int j = 0; j < U; ++j) {
-        const MixedTerm m = term_mixed(P, Q[j], m2hinv, hf, hhf, hh);
-        sw += m.dw;
-        s0 = __builtin_fmaf(m.coef, m.dx, s0);
-        s1 = __builtin_fmaf(m.coef, m.dy, s1);
-        if (DIM3) s2 = __builtin_fmaf(m.coef, m.dz, s2);
-      }
+      for (int j = 0; j < U; ++j)
+        pair_mixed<DIM3>(P, Q[j].x, Q[j].y, Q[j].z, (float)Q[j].w, K, sw, f0, f1, f2);
       de += (double)sw;
-      f0 += (double)s0;
-      f1 += (double)s1;
-      if (DIM3) f2 += (double)s2;
     } else {
 #pragma unroll
       for (int j = 0; j < U; ++j)
@@ -186,8 +192,8 @@ __device__ __forceinline__ void finish_cell(const TypeDev &T, double &de, double
 template <bool MIXED, int U, int L, bool DIM3, bool BUF>
 __device__ __forceinline__ void direct_sums(const double4 *__restrict__ pos, int64_t n, int64_t cap,
                                             __amdgpu_buffer_rsrc_t rl, uint32_t lane_off, uint32_t sub,
-                                            const double4 &P, const TypeFP64 &C64, float hf, float m2hinv,
-                                            float hhf, double hh, uint32_t cnt, uint32_t wmax, double &de,
+                                            const double4 &P, const TypeFP64 &C64, const MixedC &K, uint32_t cnt,
+                                            uint32_t wmax, double &de,
                                             double &f0, double &f1, double &f2) {
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc((void *)pos, 0, BUF ? (int)(uint32_t)(32 * (n + 1)) : 0, 0x00020000);
@@ -209,7 +215,7 @@ __device__ __forceinline__ void direct_sums(const double4 *__restrict__ pos, int
     }
   };
   auto fetch = [&](uint32_t q) -> double4 { return gather<BUF>(pos, rs, q); };
-  cell_sums<MIXED, U, L, DIM3>(P, C64, hf, m2hinv, hhf, hh, cnt, wmax, far, slot, fetch, de, f0, f1, f2);
+  cell_sums<MIXED, U, L, DIM3>(P, C64, K, cnt, wmax, far, slot, fetch, de, f0, f1, f2);
 }
 
 // descriptor over a list array from element e0 on (reads past its end return 0)
@@ -267,8 +273,7 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
   const double4 P = pos[p];
   const TypeDev &T = types[ctype_s[p]];
   const TypeFP64 C64{T.h, T.hinv, T.sc, T.size, T.sinv, T.cW, T.cF};
-  const float hf = (float)T.h, m2hinv = -2.0f * T.hinv_f, hhf = hf * hf;
-  const double hh = T.h * T.h;
+  const MixedC K = mixed_consts(T);
   const uint32_t cnt = live ? min((uint32_t)cap, count[p]) : 0u;
   const uint32_t wmax = wave_max(cnt);
   // The lists of this workgroup's cells lie in the slot groups from the one
@@ -278,7 +283,7 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
   const __amdgpu_buffer_rsrc_t rl = list_rsrc(nbr, g0 * cap * 64, nbr_elems);
   const uint32_t lane_off = (uint32_t)((((p >> 6) - g0) * cap * 64 + (p & 63)) * 4);
   double de = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  direct_sums<MIXED, U, L, DIM3, BUF>(pos, n, cap, rl, lane_off, sub, P, C64, hf, m2hinv, hhf, hh, cnt, wmax, de,
+  direct_sums<MIXED, U, L, DIM3, BUF>(pos, n, cap, rl, lane_off, sub, P, C64, K, cnt, wmax, de,
                                       f0, f1, f2);
   if (L > 1) {
 #pragma unroll
@@ -295,6 +300,14 @@ __global__ __launch_bounds__(256) void k_sph(const double4 *__restrict__ pos, co
   fx[p] = f0;
   fy[p] = f1;
   fz[p] = f2;
+}
+
+// The size half of a staged record: FP64 for the drop-in; for the mixed form
+// the FP32 size in the low word (the pair reads it without a conversion).
+template <bool MIXED>
+__device__ __forceinline__ double stage_size(double size) {
+  if constexpr (MIXED) return __longlong_as_double((long long)__float_as_uint((float)size));
+  else return size;
 }
 
 // Pair kernel over LDS tiles: one workgroup of TM_TILE lanes per tile.  The
@@ -337,13 +350,13 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
       if (i < m) {
         const double4 q = pos[si[j]];
         lxy[i] = make_double2(q.x, q.y);
-        lzs[i] = make_double2(q.z, q.w);
+        lzs[i] = make_double2(q.z, stage_size<MIXED>(q.w));
       }
     }
     for (uint32_t i = t + TM_TILE * SPEC; i < m; i += TM_TILE) {
       const double4 q = pos[ti[i]];
       lxy[i] = make_double2(q.x, q.y);
-      lzs[i] = make_double2(q.z, q.w);
+      lzs[i] = make_double2(q.z, stage_size<MIXED>(q.w));
     }
     if (t == 0) {
       lxy[m] = make_double2(TM_FAR, TM_FAR);
@@ -357,8 +370,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
   const double4 P = pos[p];
   const TypeDev &T = types[ctype_s[p]];
   const TypeFP64 C64{T.h, T.hinv, T.sc, T.size, T.sinv, T.cW, T.cF};
-  const float hf = (float)T.h, m2hinv = -2.0f * T.hinv_f, hhf = hf * hf;
-  const double hh = T.h * T.h;
+  const MixedC K = mixed_consts(T);
   const uint32_t cnt = live ? min((uint32_t)cap, count[p]) : 0u;
   const uint32_t wmax = wave_max(cnt);
   double de = 0.0, f0 = 0.0, f1 = 0.0, f2 = 0.0;
@@ -376,42 +388,34 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
       if (ch + 1 < chunks) nx = __builtin_amdgcn_raw_buffer_load_b128(r16, lane_off, (int)(1024 * (ch + 1)), 0);
       const uint32_t idx[8] = {cur.x & 0xFFFFu, cur.x >> 16, cur.y & 0xFFFFu, cur.y >> 16,
                                cur.z & 0xFFFFu, cur.z >> 16, cur.w & 0xFFFFu, cur.w >> 16};
-      float sw = 0.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+      float sw = 0.0f;
 #pragma unroll
       for (int j0 = 0; j0 < 8; j0 += U) {
         if (j0 > 0 && 8 * ch + (uint32_t)j0 >= wmax) break;  // past every list of the wave
-        double4 Q[U];
+        double2 A[U], B[U];
 #pragma unroll
         for (int j = 0; j < U; ++j) {
-          const double2 a = lxy[idx[j0 + j]], b = lzs[idx[j0 + j]];
-          Q[j] = make_double4(a.x, a.y, b.x, b.y);
+          A[j] = lxy[idx[j0 + j]];
+          B[j] = lzs[idx[j0 + j]];
         }
 #pragma unroll
         for (int j = 0; j < U; ++j) {
           if constexpr (MIXED) {
-            const MixedTerm mt = term_mixed(P, Q[j], m2hinv, hf, hhf, hh);
-            sw += mt.dw;
-            s0 = __builtin_fmaf(mt.coef, mt.dx, s0);
-            s1 = __builtin_fmaf(mt.coef, mt.dy, s1);
-            s2 = __builtin_fmaf(mt.coef, mt.dz, s2);
+            const float sq = __uint_as_float((uint32_t)__double_as_longlong(B[j].y));
+            pair_mixed<true>(P, A[j].x, A[j].y, B[j].x, sq, K, sw, f0, f1, f2);
           } else {
-            if (8 * ch + (uint32_t)(j0 + j) < cnt) pair_fp64(P, Q[j], C64, true, de, f0, f1, f2);
+            if (8 * ch + (uint32_t)(j0 + j) < cnt)
+              pair_fp64(P, make_double4(A[j].x, A[j].y, B[j].x, B[j].y), C64, true, de, f0, f1, f2);
           }
         }
       }
-      if constexpr (MIXED) {
-        de += (double)sw;
-        f0 += (double)s0;
-        f1 += (double)s1;
-        f2 += (double)s2;
-      }
+      if constexpr (MIXED) de += (double)sw;
     }
   } else {
     const int64_t g0 = g * (TM_TILE / 64);
     const __amdgpu_buffer_rsrc_t rl = list_rsrc(nbr, g0 * cap * 64, nbr_elems);
     const uint32_t lane_off = (uint32_t)((((p >> 6) - g0) * cap * 64 + (p & 63)) * 4);
-    direct_sums<MIXED, U, 1, true, true>(pos, n, cap, rl, lane_off, 0, P, C64, hf, m2hinv, hhf, hh, cnt, wmax, de,
-                                         f0, f1, f2);
+    direct_sums<MIXED, U, 1, true, true>(pos, n, cap, rl, lane_off, 0, P, C64, K, cnt, wmax, de, f0, f1, f2);
   }
   // Results go out in cell order: each lane parks its cell's sums in LDS
   // (the staged records are no longer needed) and lane t writes cell t, so
