@@ -188,6 +188,33 @@ def interp_errors(tm, gen, ctx, col, field_seed=5):
     return out
 
 
+def environment_solve(tm, ctx, grid, prod):
+    """one backward-Euler step of the field on the colony's grid with the
+    deposit as its source (the reference's 7-point system and tolerance), the
+    field starting at the boundary value; diffusion scaled so the step takes
+    tens of CG iterations.  Algorithmic bytes of one CG iteration: 80 per
+    node (new direction and A p: read r, p, write p, Ap; update: read x, p,
+    r, Ap, write x, r)."""
+    G = int(grid["G"])
+    nodes = G ** 3
+    field = tm.DeviceArray.from_host(ctx, np.full(nodes, 0.1575))
+    res = float(grid["resolution"])
+    dt = 16000.0
+    D = 2.0 * res * res / dt  # z = 2
+    par = tm.Diffusion.make(D, dt, 0.1575, 0.0, 1e-12, 200)
+    ctx.diffusion_step(grid, par, field, prod)  # sizes the work arrays
+    field.set(np.full(nodes, 0.1575))
+    ctx.sync()
+    t0 = time.perf_counter()
+    it, rel = ctx.diffusion_step(grid, par, field, prod)
+    ms = (time.perf_counter() - t0) * 1e3
+    field.free()
+    per = ms / max(it, 1)
+    return {"z": D * dt / (res * res), "iterations": it, "relative_residual": rel, "ms": ms,
+            "ms_per_iteration": per, "gbs_per_iteration": 80.0 * nodes / (per * 1e-3) / 1e9,
+            "note": "wall time of the call, host checks of convergence every 8 iterations included"}
+
+
 def time_sph(tm, ctx, a, n, P):
     B = 76.0 * n + 4.0 * P
     out = {}
@@ -259,6 +286,7 @@ def run_single(a, tm, gen):
     build_ms = ctx.time_stage(tm.STAGE_BUILD, tm.MIXED, warmup=1, iters=3)
     nbr_ms = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
     cic_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
+    env = environment_solve(tm, ctx, grid, prod)
     prod.free()
     copy_gbs = copy_bandwidth(tm, ctx)
     # resident mechanics loop: every replay of the captured graph is one whole
@@ -296,6 +324,7 @@ def run_single(a, tm, gen):
         "full_step_ms": {"total": step_ms + cic_ms, "build": build_ms, "neighbors": nbr_ms,
                          "sph_mixed": out["mixed"]["ms"] if "mixed" in out else None, "cic_deposit_256^3": cic_ms},
         "mechanics_loop_ms_per_step": loop_ms,
+        "environment_solve_256^3": env,
         "copy_kernel_gbs": copy_gbs,
         "roofline": roofline(B, mixed["ms"], traffic_from_profiles()),
         "cpu_baseline": cpu,

@@ -227,6 +227,32 @@ tm_status tm_move_cells(tm_ctx *ctx, int64_t ncells, int32_t dimension, double *
  * be NULL) */
 tm_status tm_get_positions(tm_ctx *ctx, double *x, double *y, double *z);
 
+/* -------- the environment solve -------- */
+/* Backward-Euler steps of du/dt = D lap(u) - lambda u + s on the field grid,
+ * assembled as the reference assembles its HYPRE system
+ * (src/environment.c:131-511): z = D dt / res^2 (res^2 in FP32),
+ * (1 + 6z + lambda dt) u_i - z sum of the neighbours inside the grid = b_i,
+ * b_i = u_old_i + z bc for each grid face the node lies on + s_i (Dirichlet
+ * faces: the field outside the grid is bc).  Solved by conjugate gradients
+ * on the GPU to ||b - A u|| <= tol ||b|| or maxiter iterations per step
+ * (the reference: tol 1e-12, 50 iterations). */
+typedef struct tm_diffusion {
+  double diffusion; /* environment_t.diffusioncoefficient */
+  double lambda;    /* environment_t.lambdadelay */
+  double boundary;  /* environment_t.boundarycondition */
+  double dt;        /* settings.gfdt */
+  double tol;       /* relative residual */
+  int32_t maxiter;  /* per step */
+  int32_t substeps; /* backward-Euler steps in this call (the reference's numberofiters) */
+} tm_diffusion;
+/* field: DEVICE array of the grid's nodes ([i][j][k], k fastest, like
+ * environment_t.data), u at t in, u at t + substeps dt out.  source: DEVICE
+ * array of the same shape added to every step's right-hand side (e.g. the
+ * CIC deposit's production), or NULL.  *iterations: CG iterations over all
+ * steps; *residual: the last step's final relative residual. Synchronises. */
+tm_status tm_diffusion_step(tm_ctx *ctx, const tm_grid *grid, const tm_diffusion *params, const double *source,
+                            double *field, int32_t *iterations, double *residual);
+
 /* -------- measurement -------- */
 /* runs `warmup` untimed then `iters` timed launches of a stage on the
  * context's stream, bracketed by HIP events on that stream; ms_per_iter is

@@ -640,6 +640,95 @@ int64_t or_updatepositions(int64_t n, int dimension, double *x, double *y, doubl
   return removed;
 }
 
+/* src/environment.c:229-238 (matrix) and :300-355 / :440-474 (right-hand
+ * side): the reference's HYPRE assembly restated on the [i][j][k] layout */
+static double diffusion_z(float res, double D, double dt) {
+  const float r2 = res * res;
+  return D * dt / r2;
+}
+
+void or_diffusion_rhs(const int64_t ls[3], float res, double D, double lambda, double bc, double dt,
+                      const double *src, const double *uold, double *b) {
+  const int64_t lx = ls[0], ly = ls[1], lz = ls[2];
+  const double z = diffusion_z(res, D, dt), zbc = z * bc;
+  (void)lambda;
+  for (int64_t i = 0; i < lx; i++)
+    for (int64_t j = 0; j < ly; j++)
+      for (int64_t k = 0; k < lz; k++) {
+        const int64_t n = (i * ly + j) * lz + k;
+        double v = uold[n];
+        if (i == 0) v += zbc;
+        if (i == lx - 1) v += zbc;
+        if (j == 0) v += zbc;
+        if (j == ly - 1) v += zbc;
+        if (k == 0) v += zbc;
+        if (k == lz - 1) v += zbc;
+        if (src) v += src[n];
+        b[n] = v;
+      }
+}
+
+void or_diffusion_apply(const int64_t ls[3], float res, double D, double lambda, double dt, const double *u,
+                        double *Au) {
+  const int64_t lx = ls[0], ly = ls[1], lz = ls[2];
+  const double z = diffusion_z(res, D, dt);
+  const double diag = 1 + 6.0 * z + lambda * dt;
+  for (int64_t i = 0; i < lx; i++)
+    for (int64_t j = 0; j < ly; j++)
+      for (int64_t k = 0; k < lz; k++) {
+        const int64_t n = (i * ly + j) * lz + k;
+        double nb = 0.0;
+        if (i > 0) nb += u[n - ly * lz];
+        if (i < lx - 1) nb += u[n + ly * lz];
+        if (j > 0) nb += u[n - lz];
+        if (j < ly - 1) nb += u[n + lz];
+        if (k > 0) nb += u[n - 1];
+        if (k < lz - 1) nb += u[n + 1];
+        Au[n] = diag * u[n] - z * nb;
+      }
+}
+
+int or_diffusion_solve(const int64_t ls[3], float res, double D, double lambda, double bc, double dt,
+                       const double *src, double *x, double tol, int maxiter, int substeps) {
+  const int64_t n = ls[0] * ls[1] * ls[2];
+  double *b = malloc(sizeof(double) * n), *r = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
+  int total = 0;
+  for (int s = 0; s < substeps; s++) {
+    or_diffusion_rhs(ls, res, D, lambda, bc, dt, src, x, b);
+    or_diffusion_apply(ls, res, D, lambda, dt, x, Ap);
+    double rr = 0.0, bb = 0.0;
+    for (int64_t q = 0; q < n; q++) {
+      r[q] = b[q] - Ap[q];
+      p[q] = r[q];
+      rr += r[q] * r[q];
+      bb += b[q] * b[q];
+    }
+    for (int it = 0; it < maxiter && rr > tol * tol * bb; it++) {
+      or_diffusion_apply(ls, res, D, lambda, dt, p, Ap);
+      double pAp = 0.0;
+      for (int64_t q = 0; q < n; q++) pAp += p[q] * Ap[q];
+      if (!(pAp > 0.0)) break;
+      const double alpha = rr / pAp;
+      double rn = 0.0;
+      for (int64_t q = 0; q < n; q++) {
+        x[q] += alpha * p[q];
+        r[q] -= alpha * Ap[q];
+        rn += r[q] * r[q];
+      }
+      const double beta = rn / rr;
+      rr = rn;
+      for (int64_t q = 0; q < n; q++) p[q] = r[q] + beta * p[q];
+      total++;
+    }
+  }
+  free(b);
+  free(r);
+  free(p);
+  free(Ap);
+  return total;
+}
+
 double or_wtime(void) {
 #ifdef _OPENMP
   return omp_get_wtime();

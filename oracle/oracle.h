@@ -29,6 +29,11 @@
  *   or_fieldgradient     linear / quadratic fields (closed form)
  *   or_philox4x32_10     the known-answer vectors published with the
  *                        generator (Random123's kat_vectors)
+ *   or_diffusion_*       the operator against an independent stencil and
+ *                        right-hand side written with array shifts; a
+ *                        constant field with the boundary at the same value is
+ *                        reproduced; a Dirichlet sine eigenmode is divided by
+ *                        its closed-form eigenvalue
  *   or_updatepositions   hand-computed moves for chosen draws (u = 0, 1/2,
  *                        1), glibc's documented rand() sequence for srand(1),
  *                        cells placed exactly on the box faces, draw order
@@ -157,6 +162,23 @@ void or_philox_draws(int64_t n, int dimension, uint64_t seed, uint64_t step, flo
 int64_t or_updatepositions(int64_t n, int dimension, double *x, double *y, double *z, const double *size,
                            const int32_t *ctype, const float *randommove, const double *fx, const double *fy,
                            const double *fz, const float *draws, const double box[3], uint8_t *removecell);
+
+/* The environment solve's linear system, src/environment.c:131-511 (the
+ * HYPRE assembly in environment_initsystem / environment_initboundary /
+ * environment_solve), on a local grid ls[0] x ls[1] x ls[2] stored [i][j][k]
+ * with k fastest: z = D dt / (res * res) with res * res in FP32,
+ * diag = 1 + 6 z + lambda dt; A u = diag u - z (sum of the neighbours inside
+ * the grid); b = u_old + z bc for each face the node lies on (x-, x+, y-, y+,
+ * z-, z+ in that order) + src (src may be NULL). */
+void or_diffusion_rhs(const int64_t ls[3], float res, double D, double lambda, double bc, double dt,
+                      const double *src, const double *uold, double *b);
+void or_diffusion_apply(const int64_t ls[3], float res, double D, double lambda, double dt, const double *u,
+                        double *Au);
+/* substeps backward-Euler steps: each assembles b from the current field and
+ * solves A u = b by conjugate gradients from u_old to ||b - A u|| <= tol ||b||
+ * (at most maxiter iterations).  Returns the iterations over all steps. */
+int or_diffusion_solve(const int64_t ls[3], float res, double D, double lambda, double bc, double dt,
+                       const double *src, double *field, double tol, int maxiter, int substeps);
 
 /* omp_get_wtime wrapper and thread control for the timed baseline */
 double or_wtime(void);

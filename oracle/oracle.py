@@ -100,6 +100,14 @@ def lib():
         L.or_updatepositions.argtypes = [C.c_int64, C.c_int, f64p, f64p, f64p, f64p, i32p, f32p, f64p, f64p, f64p,
                                          C.c_void_p, f64p, C.c_void_p]
         L.or_updatepositions.restype = C.c_int64
+        L.or_diffusion_rhs.argtypes = [i64p, C.c_float, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                       f64p, f64p]
+        L.or_diffusion_rhs.restype = None
+        L.or_diffusion_apply.argtypes = [i64p, C.c_float, C.c_double, C.c_double, C.c_double, f64p, f64p]
+        L.or_diffusion_apply.restype = None
+        L.or_diffusion_solve.argtypes = [i64p, C.c_float, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_void_p, f64p, C.c_double, C.c_int, C.c_int]
+        L.or_diffusion_solve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -308,6 +316,39 @@ def updatepositions(x, y, z, size, ctype, randommove, fx, fy, fz, draws=None, bo
                                  None if dr is None else dr.ctypes.data_as(C.c_void_p),
                                  np.ascontiguousarray(box, np.float64), rc.ctypes.data_as(C.c_void_p))
     return x, y, z, rc, int(k)
+
+
+def _dparams(p):
+    return (float(p["diffusion"]), float(p.get("lambda", 0.0)), float(p.get("boundary", 0.0)), float(p["dt"]))
+
+
+def diffusion_rhs(ls, res, p, uold, src=None):
+    ls = np.ascontiguousarray(ls, np.int64)
+    D, lam, bc, dt = _dparams(p)
+    b = np.zeros(int(np.prod(ls)))
+    s = None if src is None else np.ascontiguousarray(src, np.float64)
+    lib().or_diffusion_rhs(ls, np.float32(res), D, lam, bc, dt, None if s is None else s.ctypes.data_as(C.c_void_p),
+                           np.ascontiguousarray(uold, np.float64).ravel(), b)
+    return b
+
+
+def diffusion_apply(ls, res, p, u):
+    ls = np.ascontiguousarray(ls, np.int64)
+    D, lam, bc, dt = _dparams(p)
+    out = np.zeros(int(np.prod(ls)))
+    lib().or_diffusion_apply(ls, np.float32(res), D, lam, dt, np.ascontiguousarray(u, np.float64).ravel(), out)
+    return out
+
+
+def diffusion_solve(ls, res, p, field, src=None, tol=1e-12, maxiter=50, substeps=1):
+    """src/environment.c's backward-Euler step(s) solved by CG; returns (field, iterations)"""
+    ls = np.ascontiguousarray(ls, np.int64)
+    D, lam, bc, dt = _dparams(p)
+    x = np.array(field, np.float64).ravel().copy()
+    s = None if src is None else np.ascontiguousarray(src, np.float64)
+    it = lib().or_diffusion_solve(ls, np.float32(res), D, lam, bc, dt,
+                                  None if s is None else s.ctypes.data_as(C.c_void_p), x, tol, maxiter, substeps)
+    return x, int(it)
 
 
 def wtime():

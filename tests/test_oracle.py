@@ -377,3 +377,98 @@ def test_updatepositions_rand_stream_order(oracle_mod):
     for u, v in zip(a[:3], b[:3]):
         assert np.array_equal(u, v)
     assert libc.rand() == 719885386  # exactly six values were consumed
+
+
+# ---- the environment solve (src/environment.c:131-511) ----
+
+def _np_apply(u, z, diag):
+    """7-point operator with the neighbours outside the grid dropped, written
+    with array shifts (independent of the oracle's loops)"""
+    nb = np.zeros_like(u)
+    nb[1:, :, :] += u[:-1, :, :]
+    nb[:-1, :, :] += u[1:, :, :]
+    nb[:, 1:, :] += u[:, :-1, :]
+    nb[:, :-1, :] += u[:, 1:, :]
+    nb[:, :, 1:] += u[:, :, :-1]
+    nb[:, :, :-1] += u[:, :, 1:]
+    return diag * u - z * nb
+
+
+def test_diffusion_operator_and_rhs(oracle_mod):
+    O = oracle_mod
+    ls = (7, 5, 6)
+    res = 3.0
+    p = dict(diffusion=2.0, lam=0.0, boundary=0.25, dt=4.0, **{"lambda": 0.1})
+    z = 2.0 * 4.0 / float(np.float32(res) * np.float32(res))
+    diag = 1 + 6.0 * z + 0.1 * 4.0
+    rng = np.random.default_rng(3)
+    u = rng.random(ls)
+    Au = O.diffusion_apply(ls, res, p, u).reshape(ls)
+    assert np.allclose(Au, _np_apply(u, z, diag), rtol=1e-15, atol=1e-15)
+    # right-hand side: z bc per face the node lies on, plus the source
+    faces = np.zeros(ls)
+    faces[0] += 1
+    faces[-1] += 1
+    faces[:, 0] += 1
+    faces[:, -1] += 1
+    faces[:, :, 0] += 1
+    faces[:, :, -1] += 1
+    src = rng.random(ls)
+    b = O.diffusion_rhs(ls, res, p, u, src).reshape(ls)
+    assert np.allclose(b, u + z * 0.25 * faces + src, rtol=1e-15, atol=1e-15)
+    assert faces.max() == 3 and faces.min() == 0
+
+
+def test_diffusion_constant_field_is_stationary(oracle_mod):
+    """with the boundary at the field's value, no decay and no source, a
+    constant field solves the system: b = A u for u constant"""
+    O = oracle_mod
+    ls = (9, 8, 10)
+    p = {"diffusion": 1.82e-5 * 1e4, "lambda": 0.0, "boundary": 0.1575, "dt": 16000.0}
+    u0 = np.full(int(np.prod(ls)), 0.1575)
+    x, it = O.diffusion_solve(ls, 2.0, p, u0)
+    assert np.allclose(x, 0.1575, rtol=1e-13, atol=0) and it <= 1
+
+
+@pytest.mark.parametrize("modes,lam", [((1, 1, 1), 0.0), ((2, 3, 1), 0.05), ((4, 1, 5), 0.0)])
+def test_diffusion_eigenmode_decay(oracle_mod, modes, lam):
+    """a Dirichlet sine mode sin(pi m (i+1)/(L+1)) per axis is an eigenvector of
+    the operator with bc = 0: one step divides it by
+    1 + lambda dt + z sum_a 2 (1 - cos(pi m_a / (L_a + 1)))"""
+    O = oracle_mod
+    ls = (12, 10, 14)
+    res = 1.5
+    p = {"diffusion": 0.3, "lambda": lam, "boundary": 0.0, "dt": 2.0}
+    z = 0.3 * 2.0 / float(np.float32(np.float32(res) * np.float32(res)))
+    axes = [np.sin(np.pi * m * (np.arange(L) + 1) / (L + 1)) for m, L in zip(modes, ls)]
+    u0 = axes[0][:, None, None] * axes[1][None, :, None] * axes[2][None, None, :]
+    mu = 1 + lam * 2.0 + z * sum(2 * (1 - np.cos(np.pi * m / (L + 1))) for m, L in zip(modes, ls))
+    x, it = O.diffusion_solve(ls, res, p, u0, tol=1e-14, maxiter=200)
+    assert np.allclose(x.reshape(ls), u0 / mu, rtol=0, atol=1e-13 * np.abs(u0).max())
+    # two substeps divide twice
+    x2, _ = O.diffusion_solve(ls, res, p, u0, tol=1e-14, maxiter=200, substeps=2)
+    assert np.allclose(x2.reshape(ls), u0 / mu ** 2, rtol=0, atol=1e-13 * np.abs(u0).max())
+
+
+def test_diffusion_solution_satisfies_system(oracle_mod):
+    """random field, source and boundary: the solution satisfies A x = b to
+    the tolerance, with A and b formed by array shifts"""
+    O = oracle_mod
+    ls = (16, 12, 11)
+    res = 0.8
+    p = {"diffusion": 0.5, "lambda": 0.02, "boundary": 0.3, "dt": 1.0}
+    rng = np.random.default_rng(7)
+    u0, src = rng.random(ls), rng.random(ls) * 0.1
+    x, it = O.diffusion_solve(ls, res, p, u0, src, tol=1e-13, maxiter=500)
+    z = 0.5 / float(np.float32(np.float32(res) * np.float32(res)))
+    diag = 1 + 6.0 * z + 0.02
+    faces = np.zeros(ls)
+    for a in range(3):
+        sl = [slice(None)] * 3
+        sl[a] = 0
+        faces[tuple(sl)] += 1
+        sl[a] = -1
+        faces[tuple(sl)] += 1
+    b = u0 + z * 0.3 * faces + src
+    r = b - _np_apply(x.reshape(ls), z, diag)
+    assert np.linalg.norm(r) <= 2e-13 * np.linalg.norm(b) and it > 5

@@ -65,6 +65,17 @@ class Move(C.Structure):
         return o
 
 
+class Diffusion(C.Structure):
+    """tm_diffusion: the environment solve's parameters"""
+    _fields_ = [("diffusion", C.c_double), ("lambda_", C.c_double), ("boundary", C.c_double), ("dt", C.c_double),
+                ("tol", C.c_double), ("maxiter", C.c_int32), ("substeps", C.c_int32)]
+
+    @classmethod
+    def make(cls, diffusion, dt, boundary=0.0, lam=0.0, tol=1e-12, maxiter=50, substeps=1):
+        return cls(float(diffusion), float(lam), float(boundary), float(dt), float(tol), int(maxiter),
+                   int(substeps))
+
+
 _lib = None
 _vp = C.c_void_p
 _i64 = C.c_int64
@@ -126,6 +137,8 @@ def _load():
         "tm_get_positions": (_i32, [_vp, _vp, _vp, _vp]),
         "tm_graph_capture_loop": (_i32, [_vp, C.c_int, C.POINTER(Move), _vp]),
         "tm_graph_removed": (_i32, [_vp, C.POINTER(_i64)]),
+        "tm_diffusion_step": (_i32, [_vp, C.POINTER(Grid), C.POINTER(Diffusion), _vp, _vp, C.POINTER(_i32),
+                                     C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -356,6 +369,17 @@ class Context:
         g = Grid.from_dict(grid)
         f = lib().tm_cic_interp_f32 if field.dtype == np.float32 else lib().tm_cic_interp
         self._check(f(self.h, C.byref(g), field.ptr, gradient.ptr, out.ptr))
+
+    def diffusion_step(self, grid, params: Diffusion, field: DeviceArray, source: DeviceArray = None):
+        """tm_diffusion_step: the environment solve on the device field;
+        returns (CG iterations, final relative residual)"""
+        g = Grid.from_dict(grid)
+        it = _i32()
+        res = C.c_double()
+        self._check(lib().tm_diffusion_step(self.h, C.byref(g), C.byref(params),
+                                            None if source is None else source.ptr, field.ptr, C.byref(it),
+                                            C.byref(res)))
+        return it.value, res.value
 
     # ---- measurement
     def time_stage(self, stage, precision=FP64, warmup=2, iters=10):

@@ -329,6 +329,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_tile_cnt);
   free_ptr(c->d_tile_cell);
   free_ptr(c->d_nbr16);
+  free_ptr(c->d_cg);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -767,6 +768,20 @@ tm_status tm_move_cells(tm_ctx *c, int64_t n, int32_t dimension, double *x, doub
   if (w[1] & TM_FLAG_CTYPE) return tm_fail(c, TM_ERR_INVALID, "ctype out of range (0..15)");
   if (removecount) *removecount += (int64_t)w[0];
   return TM_OK;
+}
+
+tm_status tm_diffusion_step(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *source, double *field,
+                            int32_t *iterations, double *residual) {
+  if (!c) return TM_ERR_INVALID;
+  if (!g || !d || !field) return tm_fail(c, TM_ERR_INVALID, "grid, parameters and field are required");
+  for (int a = 0; a < 3; ++a)
+    if (g->localsize[a] < 1) return tm_fail(c, TM_ERR_INVALID, "grid localsize must be >= 1 on every axis");
+  if (!(g->resolution > 0.0f)) return tm_fail(c, TM_ERR_INVALID, "grid resolution must be positive");
+  if (d->substeps < 0 || d->maxiter < 0 || !(d->tol >= 0.0))
+    return tm_fail(c, TM_ERR_INVALID, "substeps and maxiter must be >= 0, tol >= 0");
+  if (!(d->diffusion >= 0.0) || !(d->dt >= 0.0) || !(d->lambda >= 0.0))
+    return tm_fail(c, TM_ERR_INVALID, "diffusion, dt and lambda must be >= 0");
+  return tm_run_diffusion(c, g, d, source, field, iterations, residual);
 }
 
 tm_status tm_get_positions(tm_ctx *c, double *x, double *y, double *z) {

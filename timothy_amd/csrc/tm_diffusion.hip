@@ -1,0 +1,269 @@
+// The environment solve: one or more backward-Euler steps of
+//   du/dt = D lap(u) - lambda u + s
+// on the field grid, as the reference assembles it for HYPRE
+// (src/environment.c:131-511):
+//   z = D dt / res^2   (res^2 formed in FP32, grid_t.resolution being float),
+//   (1 + 6 z + lambda dt) u_i - z sum_{neighbours j inside the grid} u_j = b_i,
+//   b_i = u_old_i + z bc per grid face the node lies on (the six faces in the
+//         order x-, x+, y-, y+, z-, z+) + s_i,
+// i.e. Dirichlet faces with the field outside the grid held at bc.  The
+// reference solves with HYPRE's PCG (BoomerAMG-preconditioned) to a relative
+// residual of 1e-12 in at most 50 iterations; the matrix is symmetric,
+// positive definite and strongly diagonally dominant with a constant
+// diagonal, so Jacobi-preconditioned CG (plain CG scaled by that constant,
+// with the same stopping norm) reaches the same solution to that tolerance.
+//
+// Matrix-free, one lane per node, nodes in the field's layout ([i][j][k], k
+// fastest).  An iteration is two streaming kernels and two one-block
+// reductions:
+//   k_cg_dir     p_new = r + beta p_old at the node and its 6 neighbours,
+//                Ap = A p_new, partial sums of p_new . Ap      (reads r, p;
+//                writes p_new, Ap: 32 B/node when neighbours come from cache)
+//   k_cg_alpha   alpha = rr / pAp (fixed-order sum of the partials)
+//   k_cg_update  x += alpha p, r -= alpha Ap, partial sums of r . r
+//                                                             (48 B/node)
+//   k_cg_beta    beta = rr_new / rr, convergence test, iteration count
+// The scalars live in device memory; once converged, every kernel of the
+// remaining iterations returns at once, so the host launches iterations in
+// batches and reads the state once per batch.  The partial sums are added
+// in a fixed order: results do not depend on timing.
+#include "tm_internal.h"
+
+namespace {
+
+constexpr int BD = 256;
+constexpr int MAXBLK = 4096;  // partial sums per reduction
+
+struct CGState {
+  double rr, bb, pAp, alpha, beta, tol2;
+  unsigned long long iters;
+  int done;
+  int pad;
+};
+
+struct Op {
+  int64_t lx, ly, lz;
+  double diag, z;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < BD / 64; ++i) s += sh[i];
+  return s;
+}
+
+// nodes [chunk * b, chunk * (b + 1)) belong to block b (contiguous, so a
+// block's neighbour planes are mostly its own or its neighbours')
+__device__ __forceinline__ void block_range(int64_t total, int64_t chunk, int64_t &lo, int64_t &hi) {
+  lo = (int64_t)blockIdx.x * chunk;
+  hi = min(lo + chunk, total);
+}
+
+// right-hand side, residual of the initial guess, first direction
+__global__ __launch_bounds__(BD) void k_cg_init(Op A, double zbc, const double *__restrict__ src,
+                                                const double *__restrict__ x, double *__restrict__ b,
+                                                double *__restrict__ r, double *__restrict__ p, int64_t chunk,
+                                                double *__restrict__ part_rr, double *__restrict__ part_bb) {
+  __shared__ double sh[BD / 64];
+  const int64_t total = A.lx * A.ly * A.lz, sx = A.ly * A.lz, sy = A.lz;
+  int64_t lo, hi;
+  block_range(total, chunk, lo, hi);
+  double rr = 0.0, bb = 0.0;
+  for (int64_t node = lo + threadIdx.x; node < hi; node += BD) {
+    const int64_t k = node % A.lz, j = (node / A.lz) % A.ly, i = node / sx;
+    const double xc = x[node];
+    // b in the reference's order: u_old, the faces x-, x+, y-, y+, z-, z+, then s
+    double bv = xc;
+    if (i == 0) bv += zbc;
+    if (i == A.lx - 1) bv += zbc;
+    if (j == 0) bv += zbc;
+    if (j == A.ly - 1) bv += zbc;
+    if (k == 0) bv += zbc;
+    if (k == A.lz - 1) bv += zbc;
+    if (src) bv += src[node];
+    double nb = 0.0;
+    if (i > 0) nb += x[node - sx];
+    if (i < A.lx - 1) nb += x[node + sx];
+    if (j > 0) nb += x[node - sy];
+    if (j < A.ly - 1) nb += x[node + sy];
+    if (k > 0) nb += x[node - 1];
+    if (k < A.lz - 1) nb += x[node + 1];
+    const double rv = bv - (A.diag * xc - A.z * nb);
+    b[node] = bv;
+    r[node] = rv;
+    p[node] = rv;
+    rr += rv * rv;
+    bb += bv * bv;
+  }
+  const double s1 = block_sum(rr, sh);
+  __syncthreads();
+  const double s2 = block_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = s1;
+    part_bb[blockIdx.x] = s2;
+  }
+}
+
+__global__ void k_cg_start(const double *__restrict__ part_rr, const double *__restrict__ part_bb, int nblk,
+                           double tol, int maxiter, CGState *st) {
+  __shared__ double sh[BD / 64];
+  double a = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += BD) {
+    a += part_rr[i];
+    c += part_bb[i];
+  }
+  const double rr = block_sum(a, sh);
+  __syncthreads();
+  const double bb = block_sum(c, sh);
+  if (threadIdx.x == 0) {
+    st->rr = rr;
+    st->bb = bb;
+    st->tol2 = tol * tol;
+    st->beta = 0.0;
+    st->iters = 0;
+    st->done = (rr <= st->tol2 * bb || maxiter <= 0) ? 1 : 0;
+  }
+}
+
+// p_new = r + beta p_old (at the node and its neighbours), Ap = A p_new
+__global__ __launch_bounds__(BD) void k_cg_dir(Op A, const double *__restrict__ r, const double *__restrict__ pold,
+                                               double *__restrict__ pnew, double *__restrict__ Ap, int64_t chunk,
+                                               const CGState *__restrict__ st, double *__restrict__ part) {
+  __shared__ double sh[BD / 64];
+  if (st->done) return;
+  const double beta = st->beta;
+  const int64_t total = A.lx * A.ly * A.lz, sx = A.ly * A.lz, sy = A.lz;
+  int64_t lo, hi;
+  block_range(total, chunk, lo, hi);
+  double acc = 0.0;
+  for (int64_t node = lo + threadIdx.x; node < hi; node += BD) {
+    const int64_t k = node % A.lz, j = (node / A.lz) % A.ly, i = node / sx;
+    const double pc = r[node] + beta * pold[node];
+    double nb = 0.0;
+    if (i > 0) nb += r[node - sx] + beta * pold[node - sx];
+    if (i < A.lx - 1) nb += r[node + sx] + beta * pold[node + sx];
+    if (j > 0) nb += r[node - sy] + beta * pold[node - sy];
+    if (j < A.ly - 1) nb += r[node + sy] + beta * pold[node + sy];
+    if (k > 0) nb += r[node - 1] + beta * pold[node - 1];
+    if (k < A.lz - 1) nb += r[node + 1] + beta * pold[node + 1];
+    const double ap = A.diag * pc - A.z * nb;
+    pnew[node] = pc;
+    Ap[node] = ap;
+    acc += pc * ap;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_cg_alpha(const double *__restrict__ part, int nblk, CGState *st) {
+  __shared__ double sh[BD / 64];
+  if (st->done) return;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += BD) a += part[i];
+  const double pAp = block_sum(a, sh);
+  if (threadIdx.x == 0) {
+    st->pAp = pAp;
+    if (!(pAp > 0.0)) {
+      st->done = 1;  // breakdown: the residual is zero (or the operator is not positive definite)
+      st->alpha = 0.0;
+    } else {
+      st->alpha = st->rr / pAp;
+    }
+  }
+}
+
+__global__ __launch_bounds__(BD) void k_cg_update(int64_t total, double *__restrict__ x, double *__restrict__ r,
+                                                  const double *__restrict__ p, const double *__restrict__ Ap,
+                                                  int64_t chunk, const CGState *__restrict__ st,
+                                                  double *__restrict__ part) {
+  __shared__ double sh[BD / 64];
+  if (st->done) return;
+  const double alpha = st->alpha;
+  int64_t lo, hi;
+  block_range(total, chunk, lo, hi);
+  double acc = 0.0;
+  for (int64_t node = lo + threadIdx.x; node < hi; node += BD) {
+    x[node] += alpha * p[node];
+    const double rv = r[node] - alpha * Ap[node];
+    r[node] = rv;
+    acc += rv * rv;
+  }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void k_cg_beta(const double *__restrict__ part, int nblk, int maxiter, CGState *st) {
+  __shared__ double sh[BD / 64];
+  if (st->done) return;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += BD) a += part[i];
+  const double rr = block_sum(a, sh);
+  if (threadIdx.x == 0) {
+    st->beta = rr / st->rr;
+    st->rr = rr;
+    st->iters += 1;
+    if (rr <= st->tol2 * st->bb || st->iters >= (unsigned long long)maxiter) st->done = 1;
+  }
+}
+
+}  // namespace
+
+tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src, double *field,
+                           int32_t *iterations, double *residual) {
+  const int64_t lx = g->localsize[0], ly = g->localsize[1], lz = g->localsize[2];
+  const int64_t total = lx * ly * lz;
+  // work arrays: b, r, two directions, Ap; partial sums; state
+  const int64_t need = 5 * total + 2 * MAXBLK + (int64_t)((sizeof(CGState) + 7) / 8);
+  TM_TRY(tm_ensure(c, &c->d_cg, c->cap_cg, need, "diffusion work arrays"));
+  double *b = c->d_cg, *r = b + total, *p0 = r + total, *p1 = p0 + total, *Ap = p1 + total;
+  double *part = Ap + total, *part2 = part + MAXBLK;
+  CGState *st = (CGState *)(part2 + MAXBLK);
+  const int64_t nblk0 = std::min<int64_t>(MAXBLK, (total + BD - 1) / BD);
+  const int64_t chunk = (((total + nblk0 - 1) / nblk0) + BD - 1) / BD * BD;
+  const int nblk = (int)((total + chunk - 1) / chunk);
+  Op A;
+  A.lx = lx;
+  A.ly = ly;
+  A.lz = lz;
+  const float res2 = g->resolution * g->resolution;  // FP32 product, as the reference forms it
+  A.z = d->diffusion * d->dt / (double)res2;
+  A.diag = 1 + 6.0 * A.z + d->lambda * d->dt;
+  const double zbc = A.z * d->boundary;
+  int32_t it_total = 0;
+  double last_res = 0.0;
+  for (int s = 0; s < d->substeps; ++s) {
+    k_cg_init<<<nblk, BD, 0, c->stream>>>(A, zbc, src, field, b, r, p0, chunk, part, part2);
+    k_cg_start<<<1, BD, 0, c->stream>>>(part, part2, nblk, d->tol, d->maxiter, st);
+    TM_HIP(c, hipGetLastError());
+    // p0 holds r for the first direction (beta = 0); the directions then alternate
+    double *pold = p0, *pnew = p1;
+    int32_t launched = 0;
+    CGState h{};
+    for (;;) {
+      const int32_t batch = std::min<int32_t>(8, d->maxiter - launched);
+      for (int32_t q = 0; q < batch; ++q) {
+        k_cg_dir<<<nblk, BD, 0, c->stream>>>(A, r, pold, pnew, Ap, chunk, st, part);
+        k_cg_alpha<<<1, BD, 0, c->stream>>>(part, nblk, st);
+        k_cg_update<<<nblk, BD, 0, c->stream>>>(total, field, r, pnew, Ap, chunk, st, part);
+        k_cg_beta<<<1, BD, 0, c->stream>>>(part, nblk, d->maxiter, st);
+        std::swap(pold, pnew);
+      }
+      launched += batch;
+      TM_HIP(c, hipGetLastError());
+      TM_HIP(c, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      TM_HIP(c, hipStreamSynchronize(c->stream));
+      if (h.done || launched >= d->maxiter || batch == 0) break;
+    }
+    it_total += (int32_t)h.iters;
+    last_res = h.bb > 0.0 ? sqrt(h.rr / h.bb) : sqrt(h.rr);
+  }
+  if (iterations) *iterations = it_total;
+  if (residual) *residual = last_res;
+  return TM_OK;
+}

@@ -135,6 +135,10 @@ struct tm_ctx {
   uint8_t *d_tile_cell = nullptr;   // per tile: cell offset of each rank (longest list first)
   uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_nbr16_at(g*TM_TILE + r, k, nbr_alloc)
   int64_t cap_tile_idx = 0, cap_tile_cnt = 0, cap_tile_cell = 0, cap_nbr16 = 0;
+
+  // environment solve: right-hand side, residual, two directions, A p, partial sums, state
+  double *d_cg = nullptr;
+  int64_t cap_cg = 0;
   uint32_t tile_lds = 0;            // neighbour records the pair kernel stages per tile (0: no tiles built)
 
   // outputs, Morton order
@@ -216,6 +220,8 @@ tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count);
 // nothing moves when the neighbour lists of this step overflowed
 tm_status tm_run_move(tm_ctx *c, const tm_move *m, const float *draws, uint8_t *removecell, bool graph);
 tm_status tm_set_graph_step(tm_ctx *c, uint64_t step);
+tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src, double *field,
+                           int32_t *iterations, double *residual);
 tm_status tm_take_graph_removed(tm_ctx *c, int64_t *count);
 inline unsigned long long *tm_removed_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 6); }
 inline unsigned long long *tm_graph_removed_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 8); }
