@@ -640,7 +640,7 @@ int64_t or_updatepositions(int64_t n, int dimension, double *x, double *y, doubl
   return removed;
 }
 
-/* src/environment.c:229-238 (matrix) and :300-355 / :440-474 (right-hand
+/* src/environment.c:228-243 (matrix) and :309-381 / :447-483 (right-hand
  * side): the reference's HYPRE assembly restated on the [i][j][k] layout */
 static double diffusion_z(float res, double D, double dt) {
   const float r2 = res * res;
