@@ -108,7 +108,9 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * 8), "sph_lanes" lanes per cell of the mixed kernel (1 = one lane per cell),
  * "sph_balance" (1 = the FP64 kernel ranks the cells of a workgroup by list
  * length), "sph_tiles" (1 = tm_neighbors also builds LDS tiles and the 3-D
- * pair kernels read neighbours from them; default 1); read-only "nbr_alloc"
+ * pair kernels read neighbours from them; default 1), "tile_cap4" (1 = stage
+ * at most 1279 records per tile when at most one tile in twenty needs more,
+ * so four pair-kernel workgroups fit per CU; default 1); read-only "nbr_alloc"
  * (slots per cell of the current lists) and
  * "tile_lds" (records staged per tile, 0 without tiles) */
 tm_status tm_set_param(tm_ctx *ctx, const char *key, int64_t value);

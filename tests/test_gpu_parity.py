@@ -150,8 +150,10 @@ def test_tile_overflow_path(tm, oracle_mod, ctx):
     col = gen.ball_colony(6000, 99)
     col["types"] = [dict(size=1.0, h=7.0, h2=49.0, h3=343.0, h4=2401.0)]
     ctx.set_param("sph_tiles", 1)
+    ctx.set_param("tile_cap4", 0)
     res, oc, tree = run_both(tm, O, ctx, col, tm.FP64)
     assert ctx.get_param("tile_lds") == 1537
+    ctx.set_param("tile_cap4", 1)
     ref = oracle_results(tree)
     for k in ("density", "v", "mindist", "force"):
         assert np.array_equal(res[k], ref[k]), k
@@ -159,6 +161,30 @@ def test_tile_overflow_path(tm, oracle_mod, ctx):
     r = ctx.results()
     assert np.abs(r["force"] - ref["force"]).max() <= 1e-5 * np.abs(ref["force"]).max()
     assert np.array_equal(r["mindist"], ref["mindist"])
+
+
+def test_tile_occupancy_cap(tm, ctx):
+    """2M cells of the 50M-cell workload's generator (5% of radius 1.5): a few
+    tiles need more than 1279 staged records.  With the cap they take the
+    global-record path and the rest run four workgroups per CU; FP64 results
+    are bit-identical either way, mixed within its bound."""
+    col = gen.config(4, scale=0.04)
+    ctx.upload_colony(col)
+    out = {}
+    for cap in (0, 1):
+        ctx.set_param("tile_cap4", cap)
+        ctx.step(tm.FP64)
+        lds = ctx.get_param("tile_lds")
+        say(f"tile_cap4 {cap}: staged records per tile {lds}")
+        out[cap] = (lds, ctx.results())
+        ctx.sph(tm.MIXED)
+        out[cap] += (ctx.results(),)
+    assert out[0][0] > 1280 and out[1][0] == 1280
+    for k in ("density", "v", "mindist", "force"):
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+    F = out[0][1]["force"]
+    assert np.abs(out[1][2]["force"] - F).max() <= 1e-5 * np.abs(F).max()
+    ctx.set_param("tile_cap4", 1)
 
 
 @pytest.mark.parametrize("name,make", CASES)

@@ -408,6 +408,8 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->sph_balance = v ? 1 : 0;
   } else if (!strcmp(key, "sph_tiles")) {
     c->sph_tiles = v ? 1 : 0;
+  } else if (!strcmp(key, "tile_cap4")) {
+    c->tile_cap4 = v ? 1 : 0;
   } else {
     return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
   }
@@ -424,6 +426,7 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "sph_lanes")) *v = c->sph_lanes;
   else if (!strcmp(key, "sph_balance")) *v = c->sph_balance;
   else if (!strcmp(key, "sph_tiles")) *v = c->sph_tiles;
+  else if (!strcmp(key, "tile_cap4")) *v = c->tile_cap4;
   else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
   else return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);

@@ -25,6 +25,9 @@
 #define TM_TILE 256
 #define TM_TILE_MAX 1536
 #define TM_TILE_OVER 0xFFFFFFFFu
+// staged records per tile at which four pair-kernel workgroups fill a CU's
+// 160 KB of LDS (two 16-byte halves per record)
+#define TM_TILE_LDS4 1280
 // coordinate of the stand-in record pos[n] that pads neighbour lists: far
 // enough that a pair with it lies beyond every cutoff, near enough that its
 // squared distance stays finite in FP32
@@ -130,6 +133,7 @@ struct tm_ctx {
 
   // LDS tiles built from the lists (see TM_TILE)
   int sph_tiles = 1;
+  int tile_cap4 = 1;  // cap the staged records at TM_TILE_LDS4 when few tiles are larger
   uint32_t *d_tile_idx = nullptr;   // per tile: distinct neighbours, ascending (TM_TILE_MAX slots)
   uint32_t *d_tile_cnt = nullptr;   // per tile: their number, or TM_TILE_OVER
   uint8_t *d_tile_cell = nullptr;   // per tile: cell offset of each rank (longest list first)
@@ -148,7 +152,8 @@ struct tm_ctx {
   double *d_out = nullptr;
 
   // flags: [0] error bits, [1] max neighbour count, [2] pairs low, [3] pairs high,
-  // [4] largest tile, [6..7] cells newly marked by the last eager move (u64),
+  // [4] largest tile, [5] tiles above TM_TILE_LDS4 records, [6..7] cells newly
+  // marked by the last eager move (u64),
   // [8..9] the same summed over captured-loop replays, [10..11] the captured
   // loop's next Philox step, [12..13] error bits of the moves
   unsigned int *d_flags = nullptr;

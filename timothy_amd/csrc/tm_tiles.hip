@@ -142,6 +142,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   if (t == 0) {
     tile_cnt[g] = m;
     atomicMax(flags + 4, m);
+    if (m + 1 > TM_TILE_LDS4) atomicAdd(flags + 5, 1u);  // staged only without the occupancy cap
   }
   __syncthreads();
 
@@ -167,7 +168,10 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   }
 }
 
-__global__ void k_zero_one(unsigned int *p) { *p = 0; }
+__global__ void k_zero_two(unsigned int *p) {
+  p[0] = 0;
+  p[1] = 0;
+}
 
 }  // namespace
 
@@ -182,17 +186,23 @@ tm_status tm_run_tiles(tm_ctx *c, bool check) {
   TM_TRY(tm_ensure(c, &c->d_tile_cnt, c->cap_tile_cnt, nt, "tile sizes"));
   TM_TRY(tm_ensure(c, &c->d_tile_cell, c->cap_tile_cell, nt * TM_TILE, "tile ranks"));
   TM_TRY(tm_ensure(c, &c->d_nbr16, c->cap_nbr16, tm_slot_elems(nt * TM_TILE, c->nbr_alloc), "tile lists"));
-  k_zero_one<<<1, 1, 0, c->stream>>>(c->d_flags + 4);
+  k_zero_two<<<1, 1, 0, c->stream>>>(c->d_flags + 4);
   k_tiles<<<(unsigned)nt, TM_TILE, 0, c->stream>>>(n, c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_tile_idx,
                                                     c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->d_flags);
   TM_HIP(c, hipGetLastError());
   if (check) {
-    TM_HIP(c, hipMemcpyAsync(c->h_flags + 4, c->d_flags + 4, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+    TM_HIP(c, hipMemcpyAsync(c->h_flags + 4, c->d_flags + 4, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                             c->stream));
     TM_HIP(c, hipStreamSynchronize(c->stream));
     // records staged per tile: the largest tile plus the far stand-in, rounded
-    // (at least TM_TILE: the pair kernel also parks a tile's sums there)
-    c->tile_lds = (uint32_t)std::max<int64_t>(
-        TM_TILE, std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll));
+    // (at least TM_TILE: the pair kernel also parks a tile's sums there) ...
+    int64_t lds = std::max<int64_t>(TM_TILE,
+                                    std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll));
+    // ... but no more than four workgroups' worth per CU when at most one tile
+    // in twenty is larger: those take the pair kernel's global-record path and
+    // every other tile runs at the higher occupancy
+    if (lds > TM_TILE_LDS4 && c->tile_cap4 && 20 * (int64_t)c->h_flags[5] <= nt) lds = TM_TILE_LDS4;
+    c->tile_lds = (uint32_t)lds;
   } else if (c->tile_lds == 0) {
     c->tile_lds = TM_TILE_MAX + 1;
   }
