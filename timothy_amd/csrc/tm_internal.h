@@ -22,7 +22,9 @@
 // TM_TILE_MAX, are staged once in LDS and the lists hold 16-bit tile-local
 // indices.  A tile with more distinct neighbours is marked TM_TILE_OVER and
 // read through the 32-bit lists.
+#ifndef TM_TILE
 #define TM_TILE 256
+#endif
 #define TM_TILE_MAX 1536
 #define TM_TILE_OVER 0xFFFFFFFFu
 // staged records per tile at which four pair-kernel workgroups fill a CU's

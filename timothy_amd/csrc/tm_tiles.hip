@@ -72,7 +72,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   // rank the cells by list length, longest first (counting sort; the order
   // among equal lengths is immaterial: each cell is still summed by one
   // lane over its own list)
-  const uint32_t b = 255u - min(cnt, 255u);
+  const uint32_t b = (uint32_t)(TS - 1) - min(cnt, (uint32_t)(TS - 1));  // lists of TS - 1 or more share a bin
   atomicAdd(&bins[b], 1u);
   __syncthreads();
   const uint32_t start = block_scan(bins[t], wsum);
