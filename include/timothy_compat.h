@@ -215,6 +215,11 @@ void patches_cells2envinit(systeminfo_t systeminfo, settings_t settings, patches
 void updatepositions(settings_t settings, celltype_t *celltype, cellsinfo_t *cellsinfo, unsigned char *removecell,
                      int64_t *removecount);
 
+/* ---- src/io.h: writevtk (src/io.c:372-516), one rank; writes
+ * vtk/step%08d.vtk (settings.step) like the reference.  The rest of io.c
+ * stays; remove writevtk from it when linking this library. ---- */
+void writevtk(systeminfo_t systeminfo, settings_t settings, cellsinfo_t cellsinfo);
+
 /* ---- src/fieldgradient.h ---- */
 void fieldgradient(systeminfo_t systeminfo, settings_t settings, environment_t **environment, grid_t *grid);
 

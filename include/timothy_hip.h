@@ -255,6 +255,20 @@ typedef struct tm_diffusion {
 tm_status tm_diffusion_step(tm_ctx *ctx, const tm_grid *grid, const tm_diffusion *params, const double *source,
                             double *field, int32_t *iterations, double *residual);
 
+/* -------- the VTK writer -------- */
+/* writevtk (src/io.c:372-516), one rank: the reference's legacy binary VTK
+ * file -- POINTS (positions as big-endian float), CELL_TYPES (1), SCALARS
+ * rank, SCALARS type (ctype), SCALARS density (the reference's constant 1.0)
+ * and VECTORS force.  tm_write_vtk writes the uploaded cells' current
+ * positions with the forces of the last tm_sph (also after tm_update_positions,
+ * before the next tm_build), in the cells' order; the body is packed on the
+ * GPU and written by the host. */
+tm_status tm_write_vtk(tm_ctx *ctx, const char *path, int32_t rank);
+/* the same from DEVICE arrays in cell order */
+tm_status tm_write_vtk_arrays(tm_ctx *ctx, const char *path, int64_t ncells, const double *x, const double *y,
+                              const double *z, const int32_t *ctype, const double *fx, const double *fy,
+                              const double *fz, int32_t rank);
+
 /* -------- measurement -------- */
 /* runs `warmup` untimed then `iters` timed launches of a stage on the
  * context's stream, bracketed by HIP events on that stream; ms_per_iter is

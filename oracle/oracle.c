@@ -10,6 +10,7 @@
 #include "oracle.h"
 
 #include <float.h>
+#include <stdio.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -727,6 +728,47 @@ int or_diffusion_solve(const int64_t ls[3], float res, double D, double lambda, 
   free(p);
   free(Ap);
   return total;
+}
+
+/* src/io.c:372-516 with swapnbyte (the host is little-endian) */
+static void put_be32(FILE *f, uint32_t v) {
+  const unsigned char b[4] = {(unsigned char)(v >> 24), (unsigned char)(v >> 16), (unsigned char)(v >> 8),
+                              (unsigned char)v};
+  fwrite(b, 1, 4, f);
+}
+static void put_f32(FILE *f, float v) {
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  put_be32(f, u);
+}
+
+int or_writevtk(const char *path, int64_t n, const double *x, const double *y, const double *z,
+                const int32_t *ctype, const double *fx, const double *fy, const double *fz, int rank) {
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  fprintf(f, "# vtk DataFile Version 2.0\nTimothy output\nBINARY\nDATASET UNSTRUCTURED_GRID");
+  fprintf(f, "\nPOINTS %lld float\n", (long long)n);
+  for (int64_t i = 0; i < n; i++) {
+    put_f32(f, (float)x[i]);
+    put_f32(f, (float)y[i]);
+    put_f32(f, (float)z[i]);
+  }
+  fprintf(f, "\nCELL_TYPES %lld\n", (long long)n);
+  for (int64_t i = 0; i < n; i++) put_be32(f, 1u);
+  fprintf(f, "\nPOINT_DATA %lld", (long long)n);
+  fprintf(f, "\nSCALARS rank integer 1\nLOOKUP_TABLE default\n");
+  for (int64_t i = 0; i < n; i++) put_be32(f, (uint32_t)rank);
+  fprintf(f, "\nSCALARS type integer 1\nLOOKUP_TABLE default\n");
+  for (int64_t i = 0; i < n; i++) put_be32(f, (uint32_t)ctype[i]);
+  fprintf(f, "\nSCALARS density float 1\nLOOKUP_TABLE default\n");
+  for (int64_t i = 0; i < n; i++) put_f32(f, 1.0f);
+  fprintf(f, "\nVECTORS force float\n");
+  for (int64_t i = 0; i < n; i++) {
+    put_f32(f, (float)fx[i]);
+    put_f32(f, (float)fy[i]);
+    put_f32(f, (float)fz[i]);
+  }
+  return fclose(f) == 0 ? 0 : -1;
 }
 
 double or_wtime(void) {

@@ -34,6 +34,8 @@
  *                        constant field with the boundary at the same value is
  *                        reproduced; a Dirichlet sine eigenmode is divided by
  *                        its closed-form eigenvalue
+ *   or_writevtk          byte for byte against the reference's own step-0
+ *                        output (new_run/vtk/step00000000.vtk)
  *   or_updatepositions   hand-computed moves for chosen draws (u = 0, 1/2,
  *                        1), glibc's documented rand() sequence for srand(1),
  *                        cells placed exactly on the box faces, draw order
@@ -179,6 +181,13 @@ void or_diffusion_apply(const int64_t ls[3], float res, double D, double lambda,
  * (at most maxiter iterations).  Returns the iterations over all steps. */
 int or_diffusion_solve(const int64_t ls[3], float res, double D, double lambda, double bc, double dt,
                        const double *src, double *field, double tol, int maxiter, int substeps);
+
+/* src/io.c:372-516 (writevtk), one rank: the legacy binary VTK file of the
+ * cells' positions, types and forces (big-endian; density written as the
+ * constant 1.0 like the reference).  Returns 0, or -1 if the file cannot be
+ * written. */
+int or_writevtk(const char *path, int64_t n, const double *x, const double *y, const double *z,
+                const int32_t *ctype, const double *fx, const double *fy, const double *fz, int rank);
 
 /* omp_get_wtime wrapper and thread control for the timed baseline */
 double or_wtime(void);

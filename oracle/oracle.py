@@ -108,6 +108,8 @@ def lib():
         L.or_diffusion_solve.argtypes = [i64p, C.c_float, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_void_p, f64p, C.c_double, C.c_int, C.c_int]
         L.or_diffusion_solve.restype = C.c_int
+        L.or_writevtk.argtypes = [C.c_char_p, C.c_int64, f64p, f64p, f64p, i32p, f64p, f64p, f64p, C.c_int]
+        L.or_writevtk.restype = C.c_int
         _lib = L
     return _lib
 
@@ -349,6 +351,17 @@ def diffusion_solve(ls, res, p, field, src=None, tol=1e-12, maxiter=50, substeps
     it = lib().or_diffusion_solve(ls, np.float32(res), D, lam, bc, dt,
                                   None if s is None else s.ctypes.data_as(C.c_void_p), x, tol, maxiter, substeps)
     return x, int(it)
+
+
+def writevtk(path, x, y, z, ctype, force, rank=0):
+    """src/io.c:372-516 for one rank; returns the file's bytes"""
+    f = np.asarray(force, np.float64)
+    a = lambda v: np.ascontiguousarray(v, np.float64)
+    rc = lib().or_writevtk(os.fsencode(path), len(x), a(x), a(y), a(z), np.ascontiguousarray(ctype, np.int32),
+                           a(f[:, 0]), a(f[:, 1]), a(f[:, 2]), int(rank))
+    if rc != 0:
+        raise OSError(f"cannot write {path}")
+    return open(path, "rb").read()
 
 
 def wtime():

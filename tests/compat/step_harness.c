@@ -18,6 +18,7 @@
  *         nfields x 3G^3 gradient;  int64 walk offsets[nwalk+1], int32 walk cells
  */
 #include <stdio.h>
+#include <sys/stat.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -204,6 +205,9 @@ int main(int argc, char **argv) {
     fwrite(&next, 8, 1, m);
     fclose(m);
     free(removecell);
+    /* src/main.c's output after the update: vtk/step%08d.vtk in the working directory */
+    mkdir("vtk", 0755);
+    writevtk(si, st, ci);
   }
 
   octfree(&ci);

@@ -14,7 +14,8 @@ h2,h3,h4 keep the values initialcelltype computed from the default h=1.5
 48/(pi*h4) uses h4 = 1.5**4 while u = d/h uses h = 10.
 
 Run in a container that has the reference at /root/reference; writes
-tests/golden/vtk_step0.json.
+tests/golden/vtk_step0.json (decoded positions and forces, and the file's
+797 bytes as hex).
 """
 import json
 import os
@@ -42,6 +43,8 @@ def main():
         "celltype": {"size": 5.0, "h": 10.0, "h2": h0 * h0, "h3": h0 * h0 * h0, "h4": h0 * h0 * h0 * h0},
         "seed_cells": [[1.0, 1.0, 1.0], [5.0, 5.0, 50.0], [20.0, 20.0, 20.0]],
         "dimension": 3,
+        # the whole file, for a byte-for-byte check of a VTK writer
+        "file_hex": data.hex(),
     }
     with open(OUT, "w") as f:
         json.dump(fixture, f, indent=1)

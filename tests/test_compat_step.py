@@ -80,7 +80,7 @@ def _run(tmp_path, col, precision, move=None):
     args = [B.HARNESS, inp, out]
     if move is not None:
         args += [str(move[0]), repr(float(move[1])), str(tmp_path / "move.bin")]
-    r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=120)
+    r = subprocess.run(args, env=env, capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     got = _read_output(out, col["n"], G, len(fields), min(NWALK, col["n"]))
     if move is not None:
@@ -183,3 +183,6 @@ def test_updatepositions_after_step(oracle_mod, tmp_path):
     assert np.array_equal(got["x"], ex) and np.array_equal(got["y"], ey) and np.array_equal(got["z"], ez)
     assert np.array_equal(got["removecell"], erc) and got["removecount"] == ek == 2
     assert got["next_rand"] == libc.rand()
+    # writevtk after the update (settings.step 1): moved positions, the step's forces
+    vtk = open(tmp_path / "vtk" / "step00000001.vtk", "rb").read()
+    assert vtk == O.writevtk(str(tmp_path / "ref.vtk"), ex, ey, ez, col["ctype"], F)

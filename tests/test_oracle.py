@@ -472,3 +472,18 @@ def test_diffusion_solution_satisfies_system(oracle_mod):
     b = u0 + z * 0.3 * faces + src
     r = b - _np_apply(x.reshape(ls), z, diag)
     assert np.linalg.norm(r) <= 2e-13 * np.linalg.norm(b) and it > 5
+
+
+# ---- the VTK writer (src/io.c:372-516) ----
+
+def test_writevtk_reproduces_reference_file(oracle_mod, tmp_path):
+    """the reference's own step-0 output, rewritten from the values it holds,
+    byte for byte (decoded positions as doubles, types 0, forces)"""
+    with open(os.path.join(GOLD, "vtk_step0.json")) as f:
+        fx = json.load(f)
+    want = bytes.fromhex(fx["file_hex"])
+    n = fx["n"]
+    pos = np.array(fx["positions_after_move_f32"], np.float64)
+    frc = np.array(fx["force_f32"], np.float64)
+    got = oracle_mod.writevtk(str(tmp_path / "a.vtk"), pos[:, 0], pos[:, 1], pos[:, 2], np.zeros(n, np.int32), frc)
+    assert got == want and len(got) == 797

@@ -139,6 +139,8 @@ def _load():
         "tm_graph_removed": (_i32, [_vp, C.POINTER(_i64)]),
         "tm_diffusion_step": (_i32, [_vp, C.POINTER(Grid), C.POINTER(Diffusion), _vp, _vp, C.POINTER(_i32),
                                      C.POINTER(C.c_double)]),
+        "tm_write_vtk": (_i32, [_vp, C.c_char_p, _i32]),
+        "tm_write_vtk_arrays": (_i32, [_vp, C.c_char_p, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -380,6 +382,13 @@ class Context:
                                             None if source is None else source.ptr, field.ptr, C.byref(it),
                                             C.byref(res)))
         return it.value, res.value
+
+    def write_vtk(self, path, rank=0):
+        self._check(lib().tm_write_vtk(self.h, os.fsencode(path), rank))
+
+    def write_vtk_arrays(self, path, n, x, y, z, ctype, fx, fy, fz, rank=0):
+        self._check(lib().tm_write_vtk_arrays(self.h, os.fsencode(path), n, x.ptr, y.ptr, z.ptr, ctype.ptr, fx.ptr,
+                                              fy.ptr, fz.ptr, rank))
 
     # ---- measurement
     def time_stage(self, stage, precision=FP64, warmup=2, iters=10):

@@ -24,7 +24,7 @@ COMPAT = os.path.join(HERE, "libtimothy_compat.so")
 
 HIP_SOURCES = ["tm_api.hip", "tm_sort.hip", "tm_build.hip", "tm_neighbors.hip", "tm_tiles.hip", "tm_sph.hip",
                "tm_grid.hip", "tm_octree.hip", "tm_move.hip",
-               "tm_diffusion.hip"]
+               "tm_diffusion.hip", "tm_vtk.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

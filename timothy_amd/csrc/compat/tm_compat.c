@@ -385,6 +385,18 @@ void computeremotegradient(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltyp
  * ctype (4), three draws (12), removecell (1) */
 #define MV_BYTES 73
 
+static void ensure_mv(systeminfo_t si, tm_ctx *c, int64_t n) {
+  if (n <= S.mv_cap) return;
+  free(S.mv_host);
+  if (S.mv_dev) tm_free(c, S.mv_dev);
+  S.mv_dev = NULL;
+  S.mv_cap = 0;
+  S.mv_host = malloc((size_t)MV_BYTES * (size_t)n);
+  if (!S.mv_host) FAIL(si, "cannot allocate host staging");
+  check(si, tm_malloc(c, &S.mv_dev, (size_t)MV_BYTES * (size_t)n), "cell staging");
+  S.mv_cap = n;
+}
+
 /* src/cells.c:342-365.  The random steps are drawn from the program's rand()
  * exactly as the reference draws them (cell after cell; x, y, then z), so the
  * rest of the program sees the same rand() stream; the update itself runs on
@@ -405,16 +417,7 @@ void updatepositions(settings_t st, celltype_t *celltype, cellsinfo_t *ci, unsig
   m.box[0] = BOXSIZEX;
   m.box[1] = BOXSIZEY;
   m.box[2] = BOXSIZEZ;
-  if (n > S.mv_cap) {
-    free(S.mv_host);
-    if (S.mv_dev) tm_free(c, S.mv_dev);
-    S.mv_dev = NULL;
-    S.mv_cap = 0;
-    S.mv_host = malloc((size_t)MV_BYTES * (size_t)n);
-    if (!S.mv_host) FAIL(si, "cannot allocate host staging");
-    check(si, tm_malloc(c, &S.mv_dev, (size_t)MV_BYTES * (size_t)n), "move staging");
-    S.mv_cap = n;
-  }
+  ensure_mv(si, c, n);
   /* the same layout on host and device */
   double *hx = (double *)S.mv_host, *hy = hx + n, *hz = hy + n, *hs = hz + n, *hf = hs + n;
   int32_t *ht = (int32_t *)(hf + 3 * n);
@@ -453,6 +456,37 @@ void updatepositions(settings_t st, celltype_t *celltype, cellsinfo_t *ci, unsig
     removecell[i] = hr[i];
   }
   *removecount += added;
+}
+
+/* src/io.c:372-516, one rank: vtk/step%08d.vtk from the host cells and
+ * forces, packed on the GPU (positions and forces to big-endian float) */
+void writevtk(systeminfo_t si, settings_t st, cellsinfo_t ci) {
+  if (si.size > 1) FAIL(si, "the MI355X cell-mechanics path serves one MPI rank");
+  const int64_t n = (int64_t)ci.localcount.n;
+  tm_ctx *c = ctx_for(si);
+  char path[64];
+  snprintf(path, sizeof(path), "vtk/step%08d.vtk", st.step);
+  if (n == 0) {
+    check(si, tm_write_vtk_arrays(c, path, 0, NULL, NULL, NULL, NULL, NULL, NULL, NULL, si.rank), "writevtk");
+    return;
+  }
+  ensure_mv(si, c, n);
+  /* the staging block's x, y, z and force columns, and its types */
+  double *hx = (double *)S.mv_host, *hy = hx + n, *hz = hy + n, *hs = hz + n, *hf = hs + n;
+  int32_t *ht = (int32_t *)(hf + 3 * n);
+  double *dx = (double *)S.mv_dev, *dy = dx + n, *dz = dy + n, *ds = dz + n, *df = ds + n;
+  int32_t *dt = (int32_t *)(df + 3 * n);
+  for (int64_t i = 0; i < n; i++) {
+    hx[i] = ci.cells[i].x;
+    hy[i] = ci.cells[i].y;
+    hz[i] = ci.cells[i].z;
+    hf[i] = ci.forces[i].x;
+    hf[n + i] = ci.forces[i].y;
+    hf[2 * n + i] = ci.forces[i].z;
+    ht[i] = ci.cells[i].ctype;
+  }
+  check(si, tm_memcpy_htod(c, S.mv_dev, S.mv_host, (size_t)(60 * n)), "writevtk upload");
+  check(si, tm_write_vtk_arrays(c, path, n, dx, dy, dz, dt, df, df + n, df + 2 * n, si.rank), "writevtk");
 }
 
 /* ------------------------------------------------------------------ */

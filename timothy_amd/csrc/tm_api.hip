@@ -330,6 +330,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_tile_cell);
   free_ptr(c->d_nbr16);
   free_ptr(c->d_cg);
+  free_ptr(c->d_vtk);
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -842,6 +843,7 @@ tm_status tm_graph_launch(tm_ctx *c) {
     TM_TRY(capture(c, c->graph_prec));
   }
   c->stage = 4;
+  c->sums_valid = true;
   c->moved = c->graph_loop;
   c->last_prec = c->graph_prec;
   return TM_OK;

@@ -391,6 +391,7 @@ tm_status tm_run_build(tm_ctx *c) {
   const int64_t n = c->n;
   if (c->stage < 1) return tm_fail(c, TM_ERR_STATE, "tm_build before tm_upload");
   c->moved = false;  // the sorted records are read afresh from the cells
+  c->sums_valid = false;
   if (n == 0) {
     c->stage = 2;
     return TM_OK;

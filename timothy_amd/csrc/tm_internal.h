@@ -163,6 +163,11 @@ struct tm_ctx {
 
   int stage = 0;  // 0 empty, 1 cells, 2 built, 3 neighbours, 4 sph
   bool moved = false;  // a captured loop moved the cells after its sums (cleared by the next build)
+  bool sums_valid = false;  // d_fx.. hold sums in the current d_perm order (set by the sums, cleared by a build)
+
+  // VTK body (40 bytes per cell)
+  uint32_t *d_vtk = nullptr;
+  int64_t cap_vtk = 0;
   tm_precision last_prec = TM_FP64;
 
   // last deposit, for timing

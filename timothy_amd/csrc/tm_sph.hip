@@ -515,12 +515,14 @@ tm_status tm_run_sph(tm_ctx *c, tm_precision prec) {
   if (c->stage < 3) return tm_fail(c, TM_ERR_STATE, "tm_sph before tm_neighbors");
   if (c->n == 0) {
     c->stage = 4;
+    c->sums_valid = true;
     return TM_OK;
   }
   if (prec == TM_MIXED) launch_sph<true>(c);
   else launch_sph<false>(c);
   TM_HIP(c, hipGetLastError());
   c->stage = 4;
+  c->sums_valid = true;
   c->last_prec = prec;
   return TM_OK;
 }
