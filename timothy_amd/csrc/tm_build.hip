@@ -327,43 +327,6 @@ __global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, i
   nodes[i] = r;
 }
 
-// escape links of the descending walk for every node (internal and leaf),
-// and the leaf octants: after the subtree covering [a, b], the walk goes to
-// the left child of the split between a-1 and a, which is internal node a-1
-// when that node ends at a-1, else leaf a-1; nothing after a colony's first cell.
-__global__ __launch_bounds__(BT) void k_escape(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
-                                               const int2 *__restrict__ range, int64_t n,
-                                               NodeRec *__restrict__ nodes) {
-  int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
-  if (k >= 2 * n - 1) return;
-  int64_t a;
-  if (k < n - 1) {
-    a = range[k].x;
-  } else {
-    a = k - (n - 1);
-    uint4 c = code[a];
-    int s = span[a];
-    uint32_t side = (s >= 29) ? (1u << 29) : (1u << s);
-    NodeRec r;
-    r.lo[0] = c.x & ~(side - 1u);
-    r.lo[1] = c.y & ~(side - 1u);
-    r.lo[2] = c.z & ~(side - 1u);
-    r.hi[0] = r.lo[0] + side;
-    r.hi[1] = r.lo[1] + side;
-    r.hi[2] = r.lo[2] + side;
-    r.child = (uint32_t)a;
-    r.esc = TM_END;
-    nodes[k] = r;
-  }
-  uint32_t esc = TM_END;
-  if (a > 0 && code[a - 1].w == code[a].w) {
-    int64_t j = a - 1;
-    if (j <= n - 2 && range[j].y == j) esc = (uint32_t)j;
-    else esc = (uint32_t)((n - 1) + j);
-  }
-  nodes[k].esc = esc;
-}
-
 __global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 *__restrict__ range, int64_t n,
                           int32_t *__restrict__ root, unsigned int *__restrict__ flags) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,6 +341,87 @@ __global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 
     else atomicOr(flags, (unsigned)TM_FLAG_ROOT);
   }
   root[c] = r;
+}
+
+// The walk's tree.  Each colony's radix tree is laid out in the order the
+// descending walk visits it (the first-visited child -- the one holding the
+// higher keys -- right after its parent), so a node that overlaps the query
+// box is followed by the next record and a node that does not by its escape
+// link: one 16-byte record per visit, and runs of hits read consecutive
+// records.  Colony c (cells [s, e]) takes the slots [2s, 2e + 1).  In that
+// order a node covering cells [f, l] sits at
+//   2s + 2 (e - l) + (its ancestors up to the colony root of which it lies in
+//                     the first-visited child)
+// -- the nodes before it are those ancestors and, for each ancestor it lies
+// in the second child of, that ancestor's first child with its 2 k - 1 nodes
+// over the k cells above l -- and its subtree ends 2 (l - f + 1) - 1 slots on.
+// A record holds the node's region as tm_pack-style codes (lower corner, the
+// size level in the spare top bits, bit 31 of y set for a leaf) and, in .w,
+// the escape slot of an internal node or the cell of a leaf (whose escape is
+// always the next slot).
+__global__ __launch_bounds__(BT) void k_wparents(const int2 *__restrict__ range, const int32_t *__restrict__ split,
+                                                 int64_t n, int32_t *__restrict__ par) {
+  const int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (k >= n - 1) return;
+  const int2 r = range[k];
+  const int32_t g = split[k];
+  const int64_t left = (r.x == g) ? (n - 1) + g : g;
+  const int64_t right = (r.y == g + 1) ? (n - 1) + g + 1 : g + 1;
+  par[left] = (int32_t)(k << 1);
+  par[right] = (int32_t)((k << 1) | 1);  // the child the descending walk visits first
+}
+
+__global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
+                                               const int2 *__restrict__ range, const int32_t *__restrict__ delta,
+                                               const int32_t *__restrict__ par, const int64_t *__restrict__ off,
+                                               const int32_t *__restrict__ colroot, int64_t n,
+                                               uint4 *__restrict__ wn, unsigned int *__restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (k >= 2 * n - 1) return;
+  const bool leaf = k >= n - 1;
+  int64_t f, l;
+  if (leaf) {
+    f = l = k - (n - 1);
+  } else {
+    const int2 r = range[k];
+    f = r.x;
+    l = r.y;
+  }
+  const uint4 cf = code[f];
+  const uint32_t cid = cf.w;
+  if (!leaf && code[l].w != cid) return;  // above the colonies' roots
+  const int64_t s = off[cid], e = off[cid + 1] - 1;
+  const int64_t root = colroot[cid];
+  int64_t cur = k;
+  uint32_t first = 0;
+  for (int depth = 0; cur != root; ++depth) {
+    if (depth > 256 || cur < 0 || cur >= 2 * n - 1) {  // deeper than any key allows: a malformed tree
+      atomicOr(flags, (unsigned)TM_FLAG_WALK);
+      return;
+    }
+    const int32_t p = par[cur];
+    first += (uint32_t)(p & 1);
+    cur = p >> 1;
+  }
+  const int64_t pos = 2 * s + 2 * (e - l) + first;
+  uint32_t L, rr = 0, sx, sy, sz;
+  if (leaf) {
+    L = (uint32_t)min(span[f], 29);
+    sx = sy = sz = 1u << L;
+  } else if (delta[k] >= TM_CID_BITS) {
+    const int dm = delta[k] - TM_CID_BITS;  // duplicate codes (already flagged) give dm >= 87: level 0
+    rr = (uint32_t)(dm % 3);
+    L = dm / 3 >= 29 ? 0u : 29u - (uint32_t)(dm / 3);
+    sx = 1u << L;
+    sy = sx >> (rr >= 2 ? 1 : 0);
+    sz = sx >> (rr >= 1 ? 1 : 0);
+  } else {
+    L = 29u;
+    sx = sy = sz = 0u;  // the whole code space (lower corner 0)
+  }
+  const uint32_t lx = sx ? cf.x & ~(sx - 1u) : 0u, ly = sy ? cf.y & ~(sy - 1u) : 0u, lz = sz ? cf.z & ~(sz - 1u) : 0u;
+  const uint32_t w = leaf ? (uint32_t)f : (uint32_t)(pos + 2 * (l - f + 1) - 1);
+  wn[pos] = make_uint4(lx | ((L & 7u) << 29), ly | ((L >> 3) << 29) | (leaf ? 0x80000000u : 0u), lz | (rr << 29), w);
 }
 
 __global__ void k_zero_u32(unsigned int *p, int n) {
@@ -423,8 +467,10 @@ tm_status tm_run_build(tm_ctx *c) {
   // radix tree
   if (n > 1)
     k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, n, c->d_range, c->d_delta, c->d_split, c->d_nodes);
-  k_escape<<<tm_blocks(2 * n - 1, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, n, c->d_nodes);
   k_colroot<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_col_off, c->ncol, c->d_range, n, c->d_colroot, c->d_flags);
+  if (n > 1) k_wparents<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_range, c->d_split, n, c->d_wpar);
+  k_wnodes<<<tm_blocks(2 * n - 1, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_wpar,
+                                                    c->d_col_off, c->d_colroot, n, c->d_wn, c->d_flags);
   TM_HIP(c, hipGetLastError());
   c->stage = 2;
   return TM_OK;

@@ -47,9 +47,10 @@ struct TypeDev {
   float hinv_f;
 };
 
-// 32-byte tree node: the region [lo, hi) in location-code units per axis,
-// the child visited first by the descending walk (internal nodes) and the
-// escape to the next node in that walk (TM_END after a colony's last node).
+// Internal node of the radix tree in key order: its region [lo, hi) in
+// location-code units per axis and the child the descending walk visits
+// first.  The reference-layout octree is emitted from these; the walk reads
+// the pre-ordered copy (d_wn).
 struct NodeRec {
   uint32_t lo[3];
   uint32_t child;
@@ -120,6 +121,10 @@ struct tm_ctx {
 
   // radix tree
   NodeRec *d_nodes = nullptr;
+  // the walk's tree: every colony's nodes in its descending pre-order (see
+  // tm_build.hip), and the parent links they are placed from
+  uint4 *d_wn = nullptr;
+  int32_t *d_wpar = nullptr;
   int2 *d_range = nullptr;
   int32_t *d_delta = nullptr;
   int32_t *d_split = nullptr;   // split position of each internal node

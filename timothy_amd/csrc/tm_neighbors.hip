@@ -1,7 +1,9 @@
 // Neighbour stage: one lane per cell walks its colony's radix tree with the
 // reference's query box and strict overlap test, visiting leaf octants in
 // descending Morton key -- the order the reference's LIFO DFS pops them
-// (children pushed 0..7, src/potential.c:111-131).
+// (children pushed 0..7, src/potential.c:111-131).  The tree is read in the
+// walk's own pre-order (built by k_wnodes): one 16-byte record per visited
+// node, the next slot after a hit and the escape slot after a miss.
 //
 // Per cell p this yields:
 //   N(p)    candidates q != p with sqrt((dx*dx+dy*dy)+dz*dz) <= h_p, stored in
@@ -47,8 +49,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
                                                   const int32_t *__restrict__ ctype_s,
                                                   const float *__restrict__ young_s,
-                                                  const NodeRec *__restrict__ nodes,
-                                                  const int32_t *__restrict__ colroot,
+                                                  const uint4 *__restrict__ wn,
+                                                  const int64_t *__restrict__ col_off,
                                                   const double4 *__restrict__ aff,
                                                   const TypeDev *__restrict__ types, int64_t n, int64_t cap,
                                                   uint32_t *__restrict__ nbr,
@@ -68,16 +70,13 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   const uint32_t mn0 = box_code(P.x - h, A.x, A.w), mx0 = box_code(P.x + h, A.x, A.w);
   const uint32_t mn1 = box_code(P.y - h, A.y, A.w), mx1 = box_code(P.y + h, A.y, A.w);
   const uint32_t mn2 = box_code(P.z - h, A.z, A.w), mx2 = box_code(P.z + h, A.z, A.w);
-  const uint32_t leaf0 = (uint32_t)(n - 1);
-  const uint32_t nnodes = (uint32_t)(2 * n - 1);
-  uint32_t node = live ? (uint32_t)colroot[cid] : TM_END;
+  // the colony's slots of the pre-ordered tree
+  uint32_t node = live ? (uint32_t)(2 * col_off[cid]) : 0u;
+  const uint32_t end = live ? (uint32_t)(2 * col_off[cid + 1] - 1) : 0u;
   uint32_t cnt = 0;
   int32_t qs = -1;
   double4 Qs = P;  // record of the last candidate visited
   double md2 = 1.0 / 0.0;
-  // a well-formed walk visits each node at most once; the budget only stops a
-  // malformed tree from looping
-  int64_t budget = 2 * n + 8;
   // candidate q (its leaf octant overlaps the box): distance, mindist, the
   // last-visited candidate, and the list entry when within the cutoff
   auto visit = [&](uint32_t q) {
@@ -93,20 +92,23 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
       ++cnt;
     }
   };
-  while (node != TM_END) {
-    if (node >= nnodes || --budget < 0) {
+  // One record per node: after a hit the next slot (the first child, or for
+  // a leaf the next node), after a miss the escape slot; slots only grow, so
+  // the walk ends at the colony's last slot.
+  while (node < end) {
+    const uint4 a = wn[node];
+    const uint32_t L = (a.x >> 29) | (((a.y >> 29) & 3u) << 3), r = (a.z >> 29) & 3u;
+    const uint32_t lx = a.x & 0x1FFFFFFFu, ly = a.y & 0x1FFFFFFFu, lz = a.z & 0x1FFFFFFFu;
+    const uint32_t sx = 1u << L, sy = sx >> (r >= 2u ? 1 : 0), sz = sx >> (r >= 1u ? 1 : 0);
+    const bool hit = (mx0 > lx) & (mn0 < lx + sx) & (mx1 > ly) & (mn1 < ly + sy) & (mx2 > lz) & (mn2 < lz + sz);
+    const bool leaf = (a.y & 0x80000000u) != 0u;
+    const uint32_t next = (hit || leaf) ? node + 1 : a.w;
+    if (next <= node || (leaf && a.w >= (uint32_t)n)) {  // a malformed tree: a well-formed one only moves forward
       atomicOr(flags, (unsigned)TM_FLAG_WALK);
       break;
     }
-    const uint4 a = *reinterpret_cast<const uint4 *>(&nodes[node].lo[0]);
-    const uint4 b = *reinterpret_cast<const uint4 *>(&nodes[node].hi[0]);
-    const bool hit = (mx0 > a.x) & (mn0 < b.x) & (mx1 > a.y) & (mn1 < b.y) & (mx2 > a.z) & (mn2 < b.z);
-    if (hit && node < leaf0) {
-      node = a.w;
-      continue;
-    }
-    if (hit) visit(node - leaf0);
-    node = b.w;
+    if (leaf && hit) visit(a.w);
+    node = next;
   }
   // Pad each list to the longest one of its 64-cell slot group (this wave)
   // with cell index n, the far-away stand-in record the build places after
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
 template <int BS>
 void launch_walk(tm_ctx *c) {
   k_neighbors<BS><<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
-      c->d_pos, c->d_code, c->d_ctype_s, c->d_young_s, c->d_nodes, c->d_colroot, c->d_aff, c->d_types, c->n,
+      c->d_pos, c->d_code, c->d_ctype_s, c->d_young_s, c->d_wn, c->d_col_off, c->d_aff, c->d_types, c->n,
       c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_v, c->d_flags);
 }
 
