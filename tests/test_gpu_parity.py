@@ -163,6 +163,43 @@ def test_tile_overflow_path(tm, oracle_mod, ctx):
     assert np.array_equal(r["mindist"], ref["mindist"])
 
 
+def disk_colony(side, seed):
+    """2-D colony: a jittered square lattice (spacing 1.3, jitter 0.3) cut to a
+    disk, z = 0, radius 1, h = 2 with the 2-D normalisations"""
+    rng = np.random.default_rng(seed)
+    g = (np.arange(side) - side / 2) * 1.3
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.3, 0.3, (side * side, 2))
+    pts = pts[np.hypot(pts[:, 0], pts[:, 1]) < side * 0.65 * 0.5 * 1.3 * 2 / 2]
+    n = len(pts)
+    return dict(n=n, x=pts[:, 0].copy(), y=pts[:, 1].copy(), z=np.zeros(n), size=np.ones(n),
+                young=(2100.0 + 100.0 * rng.uniform(-1, 1, n)).astype(np.float32), ctype=np.zeros(n, np.int32),
+                phase=np.ones(n, np.int32), dimension=2,
+                types=[dict(size=1.0, h=2.0, h2=4.0, h3=8.0, h4=16.0)])
+
+
+def test_sph_2d(tm, oracle_mod, ctx):
+    """a 2-D colony (src/inline.h's 2-D kernel constants, no z force): FP64
+    bit-exact against the oracle, mixed within its bound"""
+    O = oracle_mod
+    col = disk_colony(90, 4)
+    say(f"2-D disk n={col['n']}")
+    ctx.upload_colony(col)
+    ctx.step(tm.FP64)
+    r = ctx.results()
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    dens, v, md = tree.computepotential(dimension=2)
+    F = tree.computegradient(dimension=2)
+    for k, ref in (("density", dens), ("v", v), ("mindist", md), ("force", F)):
+        assert np.array_equal(r[k], ref), k
+    assert np.all(r["force"][:, 2] == 0.0) and np.abs(F[:, :2]).max() > 0
+    ctx.sph(tm.MIXED)
+    m = ctx.results()
+    assert np.abs(m["force"] - F).max() <= 1e-5 * np.abs(F).max()
+    assert np.abs(m["density"] - dens).max() <= 1e-5 * np.abs(dens).max()
+    ctx.set_celltypes(gen.CELLTYPES, 3)
+
+
 def test_tile_occupancy_cap(tm, ctx):
     """2M cells of the 50M-cell workload's generator (5% of radius 1.5): a few
     tiles need more than 1279 staged records.  With the cap they take the
