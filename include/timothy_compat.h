@@ -169,6 +169,29 @@ typedef struct environment_t {
   double *data, *gradient, *production;
 } environment_t;
 
+/* src/global.h:366-393 (built with HYPRE): its handles as opaque pointers,
+ * HYPRE_Int and HYPRE_SStructVariable as int (HYPRE's default 32-bit ints) */
+typedef struct solversettings_t {
+  double *z;
+  double *dt;
+  int enviter;
+  int numberofiters;
+  int envobjecttype;
+  int *vartypes;
+  void **stencil;
+  void *precond;
+  void *grid;
+  void *graph;
+  int lower[3];
+  int upper[3];
+  int bclower[3];
+  int bcupper[3];
+} solversettings_t;
+
+typedef struct solverdata_t {
+  void *A, *b, *x, *parA, *parb, *parx, *solver;
+} solverdata_t;
+
 /* src/global.h:408-412 */
 typedef struct octheap_t { int size, count; int *data; } octheap_t;
 
@@ -220,6 +243,21 @@ void updatepositions(settings_t settings, celltype_t *celltype, cellsinfo_t *cel
  * stays; remove writevtk from it when linking this library. ---- */
 void writevtk(systeminfo_t systeminfo, settings_t settings, cellsinfo_t cellsinfo);
 
+/* ---- src/environment.h (src/environment.c): the environment solve on the
+ * GPU instead of HYPRE.  environment_compute assembles each field's system as
+ * the reference does (right-hand side from initialconditionmean, the six
+ * Dirichlet faces and the point source of environment_pcfunction) and solves
+ * it to the reference's tolerance (1e-12, 50 iterations, numberofiters
+ * steps).  Like the reference it leaves environment[f].data unchanged; the
+ * solution is kept and tm_compat_environment_solution returns it. ---- */
+void environment_allocate(systeminfo_t systeminfo, settings_t settings, grid_t grid, environment_t **environment,
+                          solverdata_t *solverdata, solversettings_t *solversettings);
+void environment_init(systeminfo_t systeminfo, settings_t settings, grid_t grid, environment_t **environment);
+void environment_initsystem(systeminfo_t systeminfo, settings_t settings, grid_t *grid, environment_t **environment,
+                            solverdata_t *solverdata, solversettings_t *solversettings);
+void environment_compute(systeminfo_t systeminfo, settings_t settings, grid_t *grid, environment_t **environment,
+                         solverdata_t *solverdata, solversettings_t *solversettings);
+
 /* ---- src/fieldgradient.h ---- */
 void fieldgradient(systeminfo_t systeminfo, settings_t settings, environment_t **environment, grid_t *grid);
 
@@ -233,6 +271,12 @@ void tm_compat_set_device(int device);
 void tm_compat_set_host_octree(int enable);
 /* the implicit context (a tm_ctx*, see timothy_hip.h) */
 void *tm_compat_context(void);
+/* the last environment_compute's solution of field f (localsize nodes,
+ * [i][j][k] like environment_t.data) copied to out; 0 on success, -1 if there
+ * is none.  The reference computes this solution and does not keep it. */
+int tm_compat_environment_solution(int field, double *out);
+/* CG iterations of the last environment_compute (all fields and steps) */
+int tm_compat_environment_iterations(void);
 /* release the implicit context */
 void tm_compat_shutdown(void);
 

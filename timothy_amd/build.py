@@ -76,17 +76,21 @@ def build(verbose: bool = False) -> None:
 HARNESS = os.path.join(ROOT, "tests", "compat", "_build", "step_harness")
 
 
+ENV_HARNESS = os.path.join(ROOT, "tests", "compat", "_build", "env_harness")
+
+
 def build_step_harness() -> None:
-    """The C program the drop-in tests run: it links libtimothy_compat.so the
+    """The C programs the drop-in tests run: they link libtimothy_compat.so the
     way the reference's own executable would link octree.o/potential.o/..."""
-    src = os.path.join(ROOT, "tests", "compat", "step_harness.c")
-    if not os.path.exists(src):
-        return
-    os.makedirs(os.path.dirname(HARNESS), exist_ok=True)
-    tmp = HARNESS + ".tmp"
-    _run(["gcc", "-std=gnu11", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp,
-          "-L" + HERE, "-ltimothy_compat", "-ltimothy_hip", "-Wl,-rpath,$ORIGIN/../../../timothy_amd", "-lm"])
-    os.replace(tmp, HARNESS)
+    for name, out in (("step_harness.c", HARNESS), ("env_harness.c", ENV_HARNESS)):
+        src = os.path.join(ROOT, "tests", "compat", name)
+        if not os.path.exists(src):
+            continue
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        tmp = out + ".tmp"
+        _run(["gcc", "-std=gnu11", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), src, "-o", tmp,
+              "-L" + HERE, "-ltimothy_compat", "-ltimothy_hip", "-Wl,-rpath,$ORIGIN/../../../timothy_amd", "-lm"])
+        os.replace(tmp, out)
 
 
 if __name__ == "__main__":

@@ -59,6 +59,11 @@ static struct {
   /* updatepositions staging (host and device, one block each) */
   void *mv_host, *mv_dev;
   int64_t mv_cap;
+  /* environment solve: one device solution per field, the source */
+  double **d_envsol, *d_envsrc;
+  int envsol_nf;
+  int64_t envsol_nodes;
+  int env_iters;
 } S;
 
 static void free_env(void);
@@ -93,7 +98,25 @@ static void configure(void) {
 static tm_ctx *ctx_for(systeminfo_t si) {
   configure();
   if (si.size > 1) FAIL(si, "the MI355X cell-mechanics path serves one MPI rank (systeminfo.size == 1)");
-  if (!S.ctx) check(si, tm_create(&S.ctx, S.device), "tm_create");
+  if (!S.ctx) {
+    /* The GPU runtime draws from rand() while it starts up.  Timothy's own
+     * random numbers (cell placement, moves, division, the initial fields)
+     * come from the same rand() stream, so the context is created -- and the
+     * runtime exercised once -- on a private generator state, and the
+     * program's state is put back untouched. */
+    static char scratch[256];
+    char *prog = initstate(1u, scratch, sizeof(scratch));
+    tm_status s = tm_create(&S.ctx, S.device);
+    if (s == TM_OK) {
+      void *p = NULL;
+      s = tm_malloc(S.ctx, &p, 64);
+      if (s == TM_OK) s = tm_memset(S.ctx, p, 0, 64);
+      if (s == TM_OK) s = tm_sync(S.ctx);
+      if (p) tm_free(S.ctx, p);
+    }
+    setstate(prog);
+    check(si, s, "tm_create");
+  }
   return S.ctx;
 }
 
@@ -134,8 +157,16 @@ void tm_compat_shutdown(void) {
     if (S.d_grad) tm_free(S.ctx, S.d_grad);
     if (S.d_interp) tm_free(S.ctx, S.d_interp);
     if (S.mv_dev) tm_free(S.ctx, S.mv_dev);
+    for (int f = 0; f < S.envsol_nf; f++)
+      if (S.d_envsol[f]) tm_free(S.ctx, S.d_envsol[f]);
+    if (S.d_envsrc) tm_free(S.ctx, S.d_envsrc);
     tm_destroy(S.ctx);
   }
+  free(S.d_envsol);
+  S.d_envsol = NULL;
+  S.d_envsrc = NULL;
+  S.envsol_nf = 0;
+  S.envsol_nodes = 0;
   free(S.mv_host);
   S.mv_host = S.mv_dev = NULL;
   S.mv_cap = 0;
@@ -488,6 +519,148 @@ void writevtk(systeminfo_t si, settings_t st, cellsinfo_t ci) {
   check(si, tm_memcpy_htod(c, S.mv_dev, S.mv_host, (size_t)(60 * n)), "writevtk upload");
   check(si, tm_write_vtk_arrays(c, path, n, dx, dy, dz, dt, df, df + n, df + 2 * n, si.rank), "writevtk");
 }
+
+/* ------------------------------------------------------------------ */
+/* environment                                                         */
+/* ------------------------------------------------------------------ */
+
+static tm_grid grid_of(const grid_t *g, int rank);
+
+/* src/environment.c:33-52 */
+void environment_allocate(systeminfo_t si, settings_t st, grid_t grid, environment_t **environment,
+                          solverdata_t *sd, solversettings_t *ss) {
+  (void)sd;
+  if (st.numberoffields == 0) return;
+  const int nf = st.numberoffields;
+  const size_t nodes = (size_t)(grid.localsize.x * grid.localsize.y * grid.localsize.z);
+  ss->dt = calloc(nf, sizeof(double));
+  ss->z = calloc(nf, sizeof(double));
+  ss->vartypes = calloc(nf, sizeof(int));
+  ss->stencil = calloc(nf, sizeof(void *));
+  if (!ss->dt || !ss->z || !ss->vartypes || !ss->stencil) FAIL(si, "cannot allocate solver settings");
+  for (int i = 0; i < nf; i++)
+    if (!((*environment)[i].data = calloc(nodes, sizeof(double)))) FAIL(si, "cannot allocate environment->data");
+  for (int i = 0; i < nf; i++)
+    if (!((*environment)[i].production = calloc(nodes, sizeof(double))))
+      FAIL(si, "cannot allocate environment->production");
+  for (int i = 0; i < nf; i++)
+    if (!((*environment)[i].gradient = calloc(nodes * 3, sizeof(double))))
+      FAIL(si, "cannot allocate environment->gradient");
+}
+
+/* src/environment.c:54-70: mean + rand()/RAND_MAX * variance per node, in the
+ * reference's order (fields, then i, j, k), so the program's rand() stream
+ * advances as it would */
+void environment_init(systeminfo_t si, settings_t st, grid_t grid, environment_t **environment) {
+  (void)si;
+  if (st.numberoffields == 0) return;
+  const int64_t lx = grid.localsize.x, ly = grid.localsize.y, lz = grid.localsize.z;
+  for (int f = 0; f < st.numberoffields; f++)
+    for (int64_t i = 0; i < lx; i++)
+      for (int64_t j = 0; j < ly; j++)
+        for (int64_t k = 0; k < lz; k++)
+          (*environment)[f].data[ly * lz * i + lz * j + k] =
+              (*environment)[f].initialconditionmean +
+              ((double)(rand()) / RAND_MAX) * (*environment)[f].initialconditionvariance;
+}
+
+/* src/environment.c:131-243: the number of steps per call and dt (the
+ * matrix itself is formed on the GPU at each solve) */
+void environment_initsystem(systeminfo_t si, settings_t st, grid_t *grid, environment_t **environment,
+                            solverdata_t *sd, solversettings_t *ss) {
+  (void)sd;
+  if (st.numberoffields == 0) return;
+  if ((st.secondsperstep / st.gfdt) > 1) {
+    const int intdiv = (int)(st.secondsperstep / st.gfdt) + 1;
+    const float a1 = st.secondsperstep / intdiv;
+    const float a2 = st.secondsperstep / (intdiv - 1);
+    if ((st.gfdt - a1) < (a2 - st.gfdt)) {
+      ss->numberofiters = intdiv;
+      for (int v = 0; v < st.numberoffields; v++) ss->dt[v] = a1;
+    } else {
+      ss->numberofiters = intdiv - 1;
+      for (int v = 0; v < st.numberoffields; v++) ss->dt[v] = a2;
+    }
+  } else {
+    ss->numberofiters = 1;
+    for (int v = 0; v < st.numberoffields; v++) ss->dt[v] = st.secondsperstep;
+  }
+  ss->enviter = 0;
+  ss->lower[0] = (int)grid->loweridx[si.rank].x;
+  ss->lower[1] = (int)grid->loweridx[si.rank].y;
+  ss->lower[2] = (int)grid->loweridx[si.rank].z;
+  ss->upper[0] = (int)grid->upperidx[si.rank].x;
+  ss->upper[1] = (int)grid->upperidx[si.rank].y;
+  ss->upper[2] = (int)grid->upperidx[si.rank].z;
+  /* src/environment.c:229-231: the matrix uses gfdt whatever the steps */
+  for (int v = 0; v < st.numberoffields; v++) {
+    ss->dt[v] = st.gfdt;
+    ss->z[v] = (*environment)[v].diffusioncoefficient * ss->dt[v] / (grid->resolution * grid->resolution);
+  }
+}
+
+/* src/environment.c:516-523 (initboundary + initsolver + solve) */
+void environment_compute(systeminfo_t si, settings_t st, grid_t *grid, environment_t **environment,
+                         solverdata_t *sd, solversettings_t *ss) {
+  (void)sd;
+  if (st.numberoffields == 0) return;
+  if (si.rank == 0) {
+    printf("Solving field.");
+    fflush(stdout);
+  }
+  tm_ctx *c = ctx_for(si);
+  const int64_t lx = grid->localsize.x, ly = grid->localsize.y, lz = grid->localsize.z;
+  const int64_t nodes = lx * ly * lz;
+  const int nf = st.numberoffields;
+  if (S.envsol_nf != nf || S.envsol_nodes != nodes) {
+    for (int f = 0; f < S.envsol_nf; f++)
+      if (S.d_envsol[f]) tm_free(c, S.d_envsol[f]);
+    if (S.d_envsrc) tm_free(c, S.d_envsrc);
+    free(S.d_envsol);
+    S.d_envsol = calloc(nf, sizeof(double *));
+    if (!S.d_envsol) FAIL(si, "cannot allocate the solution table");
+    for (int f = 0; f < nf; f++)
+      check(si, tm_malloc(c, (void **)&S.d_envsol[f], sizeof(double) * (size_t)nodes), "environment solution");
+    check(si, tm_malloc(c, (void **)&S.d_envsrc, sizeof(double) * (size_t)nodes), "environment source");
+    S.envsol_nf = nf;
+    S.envsol_nodes = nodes;
+  }
+  /* environment_pcfunction (src/environment.c:252-269): 100 at local node
+   * (5,5,5) from step 1 on, nothing else */
+  double *host = calloc((size_t)nodes, sizeof(double));
+  if (!host) FAIL(si, "cannot allocate host staging");
+  if (st.step != 0 && lx > 5 && ly > 5 && lz > 5) host[(5 * ly + 5) * lz + 5] = 100.0;
+  check(si, tm_memcpy_htod(c, S.d_envsrc, host, sizeof(double) * (size_t)nodes), "environment source");
+  tm_grid g = grid_of(grid, si.rank);
+  S.env_iters = 0;
+  for (int f = 0; f < nf; f++) {
+    /* the right-hand side starts from initialconditionmean, not from data
+     * (src/environment.c:309-323) */
+    for (int64_t q = 0; q < nodes; q++) host[q] = (*environment)[f].initialconditionmean;
+    check(si, tm_memcpy_htod(c, S.d_envsol[f], host, sizeof(double) * (size_t)nodes), "environment field");
+    tm_diffusion d;
+    d.diffusion = (*environment)[f].diffusioncoefficient;
+    d.lambda = (*environment)[f].lambdadelay;
+    d.boundary = (*environment)[f].boundarycondition;
+    d.dt = ss->dt[f];
+    d.tol = 1.0e-12;
+    d.maxiter = 50;
+    d.substeps = ss->numberofiters;
+    int32_t it = 0;
+    double rel = 0.0;
+    check(si, tm_diffusion_step(c, &g, &d, S.d_envsrc, S.d_envsol[f], &it, &rel), "environment solve");
+    S.env_iters += it;
+  }
+  ss->enviter += ss->numberofiters;
+  free(host);
+}
+
+int tm_compat_environment_solution(int field, double *out) {
+  if (!S.ctx || field < 0 || field >= S.envsol_nf || !out) return -1;
+  return tm_memcpy_dtoh(S.ctx, out, S.d_envsol[field], sizeof(double) * (size_t)S.envsol_nodes) == TM_OK ? 0 : -1;
+}
+
+int tm_compat_environment_iterations(void) { return S.env_iters; }
 
 /* ------------------------------------------------------------------ */
 /* patches: cells <-> grid                                             */
