@@ -77,26 +77,40 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   int32_t qs = -1;
   double4 Qs = P;  // record of the last candidate visited
   double md2 = 1.0 / 0.0;
-  // candidate q (its leaf octant overlaps the box): distance, mindist, the
-  // last-visited candidate, and the list entry when within the cutoff
-  auto visit = [&](uint32_t q) {
-    if (q == (uint32_t)p) return;
-    const double4 Q = pos[q];
+  // candidate q (its leaf octant overlaps the box, q != p) with its record Q:
+  // distance, mindist, the last-visited candidate, and the list entry when
+  // within the cutoff (returned as its slot, stored by the caller; -1 if none)
+  auto visit = [&](uint32_t q, const double4 &Q) -> int64_t {
     const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
     const double d2 = (dx * dx + dy * dy) + dz * dz;
     md2 = d2 < md2 ? d2 : md2;
     qs = (int32_t)q;
     Qs = Q;
+    int64_t slot = -1;
     if (d2 <= cut2) {
-      if (cnt < cap) nbr[tm_slot(p, cnt, cap)] = q;
+      if (cnt < cap) slot = tm_slot(p, cnt, cap);
       ++cnt;
     }
+    return slot;
   };
   // One record per node: after a hit the next slot (the first child, or for
   // a leaf the next node), after a miss the escape slot; slots only grow, so
-  // the walk ends at the colony's last slot.
+  // the walk ends at the colony's last slot.  A candidate's record load is
+  // issued when its leaf is found and consumed one node later, under the
+  // next node's load; its list entry is stored after that node's loads are
+  // issued, so waiting for them does not also wait for the store (vector
+  // memory counters retire in order).  Candidates are still taken in visit
+  // order.
+  uint32_t pq = 0xFFFFFFFFu;  // the candidate whose record is in flight
+  double4 Qp = P;
   while (node < end) {
     const uint4 a = wn[node];
+    int64_t slot = -1;
+    const uint32_t q = pq;
+    if (pq != 0xFFFFFFFFu) {
+      slot = visit(pq, Qp);
+      pq = 0xFFFFFFFFu;
+    }
     const uint32_t L = (a.x >> 29) | (((a.y >> 29) & 3u) << 3), r = (a.z >> 29) & 3u;
     const uint32_t lx = a.x & 0x1FFFFFFFu, ly = a.y & 0x1FFFFFFFu, lz = a.z & 0x1FFFFFFFu;
     const uint32_t sx = 1u << L, sy = sx >> (r >= 2u ? 1 : 0), sz = sx >> (r >= 1u ? 1 : 0);
@@ -104,11 +118,20 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
     const bool leaf = (a.y & 0x80000000u) != 0u;
     const uint32_t next = (hit || leaf) ? node + 1 : a.w;
     if (next <= node || (leaf && a.w >= (uint32_t)n)) {  // a malformed tree: a well-formed one only moves forward
+      if (slot >= 0) nbr[slot] = q;
       atomicOr(flags, (unsigned)TM_FLAG_WALK);
       break;
     }
-    if (leaf && hit) visit(a.w);
+    if (leaf && hit && a.w != (uint32_t)p) {
+      Qp = pos[a.w];
+      pq = a.w;
+    }
+    if (slot >= 0) nbr[slot] = q;
     node = next;
+  }
+  if (pq != 0xFFFFFFFFu) {
+    const int64_t slot = visit(pq, Qp);
+    if (slot >= 0) nbr[slot] = pq;
   }
   // Pad each list to the longest one of its 64-cell slot group (this wave)
   // with cell index n, the far-away stand-in record the build places after
