@@ -82,19 +82,27 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   order[rank] = (uint8_t)t;
   rcnt[rank] = cnt;
 
-  // distinct neighbours of the tile (stop counting past TM_TILE_MAX)
-  for (uint32_t k = 0; k < cnt; ++k) {
+  // distinct neighbours of the tile (stop counting past TM_TILE_MAX); a
+  // lane reads 8 list slots per round so their loads overlap
+  for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
     if (*(volatile uint32_t *)&distinct > TM_TILE_MAX) break;
-    const uint32_t q = nbr[tm_slot(cell, k, cap)];
-    uint32_t h = hslot(q);
-    for (;;) {
-      const uint32_t old = atomicCAS(&keys[h], EMPTY, q);
-      if (old == EMPTY) {
-        atomicAdd(&distinct, 1u);
-        break;
+    uint32_t qs[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qs[j] = k0 + j < cnt ? nbr[tm_slot(cell, k0 + j, cap)] : EMPTY;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t q = qs[j];
+      if (q == EMPTY) continue;
+      uint32_t h = hslot(q);
+      for (;;) {
+        const uint32_t old = atomicCAS(&keys[h], EMPTY, q);
+        if (old == EMPTY) {
+          atomicAdd(&distinct, 1u);
+          break;
+        }
+        if (old == q) break;
+        h = (h + 1) & (HASH - 1);
       }
-      if (old == q) break;
-      h = (h + 1) & (HASH - 1);
     }
   }
   __syncthreads();
@@ -156,15 +164,24 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   tile_cell[g * TS + t] = (uint8_t)off;
   const int64_t r = g * TS + t;
   const uint32_t pad8 = (pad + 7) & ~7u;  // <= cap, a multiple of 8
-  for (uint32_t k = 0; k < pad8; ++k) {
-    uint16_t v = (uint16_t)m;
-    if (k < rc) {
-      const uint32_t q = nbr[tm_slot(src, k, cap)];
-      uint32_t h = hslot(q);
-      while (keys[h] != q) h = (h + 1) & (HASH - 1);
-      v = vals[h];
+  // one 16-byte chunk of 8 slots per round: its 8 list loads overlap
+  for (uint32_t k0 = 0; k0 < pad8; k0 += 8) {
+    uint32_t qs[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qs[j] = k0 + j < rc ? nbr[tm_slot(src, k0 + j, cap)] : EMPTY;
+    uint32_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      v[j] = m;
+      const uint32_t q = qs[j];
+      if (q != EMPTY) {
+        uint32_t h = hslot(q);
+        while (keys[h] != q) h = (h + 1) & (HASH - 1);
+        v[j] = vals[h];
+      }
     }
-    nbr16[tm_nbr16_at(r, k, cap)] = v;
+    *reinterpret_cast<uint4 *>(nbr16 + tm_nbr16_at(r, k0, cap)) =
+        make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
   }
 }
 
