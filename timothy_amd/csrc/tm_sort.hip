@@ -37,10 +37,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
   for (int j = tid; j < RS_WAVES * 256; j += RS_THREADS) (&h[0][0])[j] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
+  uint32_t dg[RS_ITEMS];  // all of this lane's digits are loaded before any is counted
+#pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    int64_t i = base + it * 64 + lane;
-    bool valid = i < n;
-    uint32_t d = valid ? (uint32_t)((keys[i] >> shift) & 255u) : 0u;
+    const int64_t i = base + it * 64 + lane;
+    dg[it] = i < n ? (uint32_t)((keys[i] >> shift) & 255u) : 0u;
+  }
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const bool valid = base + it * 64 + lane < n;
+    const uint32_t d = dg[it];
     unsigned long long peers = tm_match8(d, valid);
     if (valid && (peers & tm_lanemask_lt()) == 0) h[w][d] += (uint32_t)__popcll(peers);
   }
@@ -51,6 +57,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
   hist[(int64_t)tid * nblocks + blockIdx.x] = s;
 }
 
+// The tile's keys are ranked stably (by digit, then input order), placed in
+// LDS in that order, and written out from there: consecutive lanes write
+// consecutive slots of one digit's run instead of 64 scattered records.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ keys,
                                                            const int32_t *__restrict__ vals,
                                                            uint64_t *__restrict__ okeys, int32_t *__restrict__ ovals,
@@ -59,20 +68,29 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
   __shared__ uint32_t cnt[RS_WAVES][256];
   __shared__ uint32_t pre[RS_WAVES][256];
   __shared__ uint32_t gbase[256];
+  __shared__ uint32_t tstart[256];  // first tile-local slot of each digit
+  __shared__ uint32_t wsum[RS_WAVES];
+  __shared__ uint64_t skey[RS_TILE];
+  __shared__ int32_t sval[RS_TILE];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int j = tid; j < RS_WAVES * 256; j += RS_THREADS) (&cnt[0][0])[j] = 0;
   gbase[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
-  __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
+  const int64_t tile0 = (int64_t)blockIdx.x * RS_TILE;
+  const int64_t base = tile0 + (int64_t)w * (64 * RS_ITEMS);
   uint64_t k[RS_ITEMS];
   int32_t v[RS_ITEMS];
   uint32_t rank[RS_ITEMS];
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    int64_t i = base + it * 64 + lane;
-    bool valid = i < n;
+    const int64_t i = base + it * 64 + lane;
+    const bool valid = i < n;
     k[it] = valid ? keys[i] : 0ull;
     v[it] = valid ? vals[i] : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const bool valid = base + it * 64 + lane < n;
     uint32_t d = (uint32_t)((k[it] >> shift) & 255u);
     unsigned long long peers = tm_match8(d, valid);
     uint32_t before = (uint32_t)__popcll(peers & tm_lanemask_lt());
@@ -83,6 +101,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
     rank[it] = c0 + before;
   }
   __syncthreads();
+  uint32_t total = 0;
   {
     uint32_t run = 0;
 #pragma unroll
@@ -90,18 +109,43 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
       pre[q][tid] = run;
       run += cnt[q][tid];
     }
+    total = run;
+  }
+  // exclusive scan of the digit totals over the 256 threads
+  {
+    uint32_t inc = total;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t wb = 0;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q)
+      if (q < w) wb += wsum[q];
+    tstart[tid] = wb + inc - total;
   }
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < RS_ITEMS; ++it) {
-    int64_t i = base + it * 64 + lane;
-    if (i < n) {
-      uint32_t d = (uint32_t)((k[it] >> shift) & 255u);
-      int64_t pos = (int64_t)gbase[d] + pre[w][d] + rank[it];
-      if (pos < n) {  // always true for a consistent histogram; keeps a bad scan from writing out of bounds
-        okeys[pos] = k[it];
-        ovals[pos] = v[it];
-      }
+    if (base + it * 64 + lane < n) {
+      const uint32_t d = (uint32_t)((k[it] >> shift) & 255u);
+      const uint32_t lp = tstart[d] + pre[w][d] + rank[it];  // < the tile's valid count
+      skey[lp] = k[it];
+      sval[lp] = v[it];
+    }
+  }
+  __syncthreads();
+  const int64_t valid_n = min((int64_t)RS_TILE, n - tile0);
+  for (int j = tid; j < valid_n; j += RS_THREADS) {
+    const uint64_t key = skey[j];
+    const uint32_t d = (uint32_t)((key >> shift) & 255u);
+    const int64_t pos = (int64_t)gbase[d] + (uint32_t)j - tstart[d];
+    if (pos >= 0 && pos < n) {  // always true for a consistent histogram; keeps a bad scan from writing out of bounds
+      okeys[pos] = key;
+      ovals[pos] = sval[j];
     }
   }
 }
