@@ -384,6 +384,83 @@ def test_batched_colonies_match_single(tm, oracle_mod, ctx):
         assert np.array_equal(sh, t.shift) and sc == t.scale
 
 
+def test_batched_with_degenerate_colonies(tm, oracle_mod, ctx):
+    """empty, one-cell and two-cell colonies between ordinary ones: every
+    multi-cell colony matches its own oracle run bit for bit (lists, sums),
+    the one-cell colony sums nothing, and the step is the same when graph
+    captured"""
+    O = oracle_mod
+    types = gen.CELLTYPES
+    big = gen.colonies_config3(ncol=3, scale=0.02)
+    one = dict(n=1, x=np.array([5.0]), y=np.array([5.0]), z=np.array([5.0]), size=np.array([1.0]),
+               young=np.array([2100.0], np.float32), ctype=np.array([0], np.int32), phase=np.array([1], np.int32),
+               types=types)
+    two = dict(n=2, x=np.array([0.0, 1.2]), y=np.array([0.0, 0.3]), z=np.array([0.0, -0.4]), size=np.array([1.0, 1.0]),
+               young=np.array([2100.0, 2000.0], np.float32), ctype=np.array([0, 0], np.int32),
+               phase=np.array([1, 1], np.int32), types=types)
+    empty = dict(n=0, x=np.zeros(0), y=np.zeros(0), z=np.zeros(0), size=np.zeros(0), young=np.zeros(0, np.float32),
+                 ctype=np.zeros(0, np.int32), phase=np.zeros(0, np.int32), types=types)
+    cols = [big[0], empty, one, big[1], two, empty, big[2], one]
+    say(f"degenerate batch n={[c['n'] for c in cols]}")
+    off = ctx.upload_colonies(cols)
+    ctx.step(tm.FP64)
+    res = ctx.results()
+    loff, nbr = ctx.neighbor_lists()
+    q = ctx.qstar()
+    for i, c in enumerate(cols):
+        sl = slice(off[i], off[i + 1])
+        if c["n"] == 0:
+            continue
+        if c["n"] == 1:
+            g = off[i]
+            assert loff[g + 1] == loff[g] and q[g] == -1, i
+            assert res["density"][sl][0] == 0.0 and np.all(res["force"][sl] == 0.0), i
+            continue
+        t = O.Octree(O.Colony.from_dict(c), c["types"])
+        ref = oracle_results(t)
+        for k in ("density", "v", "mindist", "force"):
+            assert np.array_equal(res[k][sl], ref[k]), (i, k)
+        coff, cand, inh = t.candidates()
+        for p in range(c["n"]):
+            g = off[i] + p
+            cc = cand[coff[p]:coff[p + 1]]
+            exp = cc[inh[coff[p]:coff[p + 1]] == 1] + off[i]
+            assert np.array_equal(nbr[loff[g]:loff[g + 1]], exp), (i, p)
+            assert q[g] == (cc[-1] + off[i] if len(cc) else -1), (i, p)
+    # the captured step reproduces the eager one
+    ctx.graph_capture(tm.FP64)
+    ctx.graph_launch()
+    ctx.sync()
+    res2 = ctx.results()
+    for k in res:
+        assert np.array_equal(res[k], res2[k]), k
+    ctx.graph_destroy()
+
+
+def test_colony_far_from_origin(tm, oracle_mod, ctx):
+    """a colony shifted far from the origin (positions of order 1e5): the
+    affine map, location codes, lists and FP64 sums still match the oracle
+    bit for bit"""
+    O = oracle_mod
+    col = dict(gen.config(1, scale=0.005))
+    col["x"] = np.asarray(col["x"]) + 1.0e5
+    col["y"] = np.asarray(col["y"]) - 3.25e4
+    col["z"] = np.asarray(col["z"]) + 7.5e3
+    res, oc, tree = run_both(tm, O, ctx, col)
+    sh, sc = ctx.affine()
+    assert np.array_equal(sh, tree.shift) and sc == tree.scale
+    assert np.array_equal(ctx.codes(), tree.loc())
+    ref = oracle_results(tree)
+    for k in ("density", "v", "mindist", "force"):
+        assert np.array_equal(res[k], ref[k]), k
+    off, nbr = ctx.neighbor_lists()
+    coff, cand, inh = tree.candidates()
+    assert int(inh.sum()) == len(nbr) and len(nbr) > 0
+    for p in range(col["n"]):
+        c = cand[coff[p]:coff[p + 1]]
+        assert np.array_equal(nbr[off[p]:off[p + 1]], c[inh[coff[p]:coff[p + 1]] == 1]), p
+
+
 def test_graph_replay_matches_eager(tm, ctx):
     col = gen.config(1, scale=0.02)
     ctx.upload_colony(col)
