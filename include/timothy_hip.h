@@ -110,7 +110,9 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * length), "sph_tiles" (1 = tm_neighbors also builds LDS tiles and the 3-D
  * pair kernels read neighbours from them; default 1), "tile_cap4" (1 = stage
  * at most 1279 records per tile when at most one tile in twenty needs more,
- * so four pair-kernel workgroups fit per CU; default 1); read-only "nbr_alloc"
+ * so four pair-kernel workgroups fit per CU; default 1), "walk_window" slots
+ * of the pre-ordered tree a 256-cell walk workgroup stages in LDS around its
+ * cells' leaves (0, 512, 1024, 2048; default 1024); read-only "nbr_alloc"
  * (slots per cell of the current lists) and
  * "tile_lds" (records staged per tile, 0 without tiles) */
 tm_status tm_set_param(tm_ctx *ctx, const char *key, int64_t value);

@@ -90,8 +90,8 @@ def test_build_and_neighbors_exact(tm, oracle_mod, ctx, name, make):
 
 
 def test_launch_shapes_invariant(tm, oracle_mod, ctx):
-    """Walk workgroup size, pair-kernel workgroup size, neighbours per trip,
-    lanes per cell and workgroup ranking change nothing: lists, qstar and
+    """Walk workgroup size and LDS window, pair-kernel workgroup size,
+    neighbours per trip, lanes per cell and workgroup ranking change nothing: lists, qstar and
     mindist stay equal to the oracle's, FP64 stays bit-exact, and the mixed
     sums stay within their bound."""
     O = oracle_mod
@@ -104,9 +104,10 @@ def test_launch_shapes_invariant(tm, oracle_mod, ctx):
     exp_q = np.array([cand[coff[p + 1] - 1] if coff[p + 1] > coff[p] else -1 for p in range(col["n"])])
     ctx.neighbors()
     lists = ctx.neighbor_lists()
-    for nb in (64, 128, 256):
-        say(f"nbr_block {nb}")
+    for nb, win in ((64, 0), (128, 0), (256, 0), (256, 512), (256, 2048), (256, 1024)):
+        say(f"nbr_block {nb} walk_window {win}")
         ctx.set_param("nbr_block", nb)
+        ctx.set_param("walk_window", win)
         ctx.neighbors()
         got = ctx.neighbor_lists()
         assert np.array_equal(got[0], lists[0]) and np.array_equal(got[1], lists[1])

@@ -415,6 +415,10 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->sph_tiles = v ? 1 : 0;
   } else if (!strcmp(key, "tile_cap4")) {
     c->tile_cap4 = v ? 1 : 0;
+  } else if (!strcmp(key, "walk_window")) {
+    if (v != 0 && v != 512 && v != 1024 && v != 2048)
+      return tm_fail(c, TM_ERR_INVALID, "walk_window in {0,512,1024,2048}");
+    c->walk_window = (int)v;
   } else {
     return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
   }
@@ -432,6 +436,7 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "sph_balance")) *v = c->sph_balance;
   else if (!strcmp(key, "sph_tiles")) *v = c->sph_tiles;
   else if (!strcmp(key, "tile_cap4")) *v = c->tile_cap4;
+  else if (!strcmp(key, "walk_window")) *v = c->walk_window;
   else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
   else return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
@@ -858,21 +863,25 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   if (c->stage < 3) TM_TRY(tm_step(c, p));
   std::string rep;
   char line[160];
-  // neighbour walk workgroup
-  int best_nb = c->nbr_block;
+  // neighbour walk workgroup and the tree window it stages in LDS
+  int best_nb = c->nbr_block, best_win = c->walk_window;
   double best_t = 1e30;
-  for (int bs : {64, 128, 256}) {
-    c->nbr_block = bs;
+  const int shapes[][2] = {{64, 0}, {128, 0}, {256, 0}, {256, 512}, {256, 1024}, {256, 2048}};
+  for (const auto &sh : shapes) {
+    c->nbr_block = sh[0];
+    c->walk_window = sh[1];
     double ms = 0;
     TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 3, &ms));
-    snprintf(line, sizeof(line), "nbr_block=%d %.4f ms\n", bs, ms);
+    snprintf(line, sizeof(line), "nbr_block=%d walk_window=%d %.4f ms\n", sh[0], sh[1], ms);
     rep += line;
     if (ms < best_t) {
       best_t = ms;
-      best_nb = bs;
+      best_nb = sh[0];
+      best_win = sh[1];
     }
   }
   c->nbr_block = best_nb;
+  c->walk_window = best_win;
   TM_TRY(tm_run_neighbors(c, true));
   // pair kernel: workgroup size x neighbours fetched per trip
   int best_sb = c->sph_block;

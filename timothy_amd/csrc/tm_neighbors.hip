@@ -45,7 +45,14 @@ __device__ __forceinline__ double pot_fp64(double4 P, float yp, double4 Q, float
   return (2.0 * (xc * xc) / (5.0 * D)) * sqrt((s1 * s2) / (s1 + s2)) + area * 0.1;
 }
 
-template <int BS>
+// WIN > 0: the workgroup first copies WIN consecutive slots of the
+// pre-ordered tree, centred on the leaves of its own cells, into LDS; its
+// walks read the nodes inside that window from LDS and the rest from memory.
+// The cells of a workgroup are consecutive in Morton order, so their walks
+// share many nodes near their own leaves (over half of all visits at 2048
+// slots for 256 cells), and a read from LDS spares the texture path the
+// per-lane gather that bounds the walk.
+template <int BS, int WIN>
 __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
                                                   const int32_t *__restrict__ ctype_s,
                                                   const float *__restrict__ young_s,
@@ -66,10 +73,31 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   const TypeDev &T = types[ctype_s[p]];
   const double h = T.h;
   const double cut2 = T.cut2;
+  const uint4 *winp = nullptr;
   // query box, src/octree.c:246-266: ((x - h) - shift) / scale * 2^29
   const uint32_t mn0 = box_code(P.x - h, A.x, A.w), mx0 = box_code(P.x + h, A.x, A.w);
   const uint32_t mn1 = box_code(P.y - h, A.y, A.w), mx1 = box_code(P.y + h, A.y, A.w);
   const uint32_t mn2 = box_code(P.z - h, A.z, A.w), mx2 = box_code(P.z + h, A.z, A.w);
+  // the window of slots staged in LDS: around the leaf slot of the
+  // workgroup's middle cell, 2 s + 2 (e - k) for cell k of colony [s, e]
+  // (up to its ancestor count), within the tree's 2 n - 1 slots
+  uint32_t wlo = 0;
+  if constexpr (WIN > 0) {
+    __shared__ uint4 win[WIN];
+    const int64_t km = min((int64_t)(tm_xcd_block() * BS + BS / 2), n - 1);
+    const uint32_t cm = code[km].w;
+    const int64_t cs = col_off[cm], ce = col_off[cm + 1] - 1;
+    const int64_t nslot = 2 * n - 1;
+    int64_t lo = 2 * cs + 2 * (ce - km) - WIN / 2;
+    lo = max((int64_t)0, min(lo, nslot - WIN));
+    wlo = (uint32_t)max((int64_t)0, lo);
+    for (int i = threadIdx.x; i < WIN; i += BS) {
+      const int64_t sl = (int64_t)wlo + i;
+      win[i] = sl < nslot ? wn[sl] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    winp = win;
+  }
   // the colony's slots of the pre-ordered tree
   uint32_t node = live ? (uint32_t)(2 * col_off[cid]) : 0u;
   const uint32_t end = live ? (uint32_t)(2 * col_off[cid + 1] - 1) : 0u;
@@ -104,7 +132,13 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   uint32_t pq = 0xFFFFFFFFu;  // the candidate whose record is in flight
   double4 Qp = P;
   while (node < end) {
-    const uint4 a = wn[node];
+    uint4 a;
+    if constexpr (WIN > 0) {
+      const uint32_t wi = node - wlo;
+      a = wi < (uint32_t)WIN ? winp[wi] : wn[node];
+    } else {
+      a = wn[node];
+    }
     int64_t slot = -1;
     const uint32_t q = pq;
     if (pq != 0xFFFFFFFFu) {
@@ -157,9 +191,9 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   if (cnt > cap) atomicMax(flags + 1, cnt);
 }
 
-template <int BS>
+template <int BS, int WIN>
 void launch_walk(tm_ctx *c) {
-  k_neighbors<BS><<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
+  k_neighbors<BS, WIN><<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
       c->d_pos, c->d_code, c->d_ctype_s, c->d_young_s, c->d_wn, c->d_col_off, c->d_aff, c->d_types, c->n,
       c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_v, c->d_flags);
 }
@@ -188,9 +222,12 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     }
     c->nbr_alloc = c->nbr_cap;
     const int bs = c->nbr_block;
-    if (bs == 64) launch_walk<64>(c);
-    else if (bs == 128) launch_walk<128>(c);
-    else launch_walk<256>(c);
+    if (bs == 64) launch_walk<64, 0>(c);
+    else if (bs == 128) launch_walk<128, 0>(c);
+    else if (c->walk_window == 2048 && 2 * n - 1 >= 2048) launch_walk<256, 2048>(c);
+    else if (c->walk_window == 1024 && 2 * n - 1 >= 1024) launch_walk<256, 1024>(c);
+    else if (c->walk_window == 512 && 2 * n - 1 >= 512) launch_walk<256, 512>(c);
+    else launch_walk<256, 0>(c);
     TM_HIP(c, hipGetLastError());
     if (!check) break;
     TM_HIP(c, hipMemcpyAsync(c->h_flags, c->d_flags, 4 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
