@@ -355,8 +355,9 @@ __global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 
 // -- the nodes before it are those ancestors and, for each ancestor it lies
 // in the second child of, that ancestor's first child with its 2 k - 1 nodes
 // over the k cells above l -- and its subtree ends 2 (l - f + 1) - 1 slots on.
-// A record holds the node's region as tm_pack-style codes (lower corner, the
-// size level in the spare top bits, bit 31 of y set for a leaf) and, in .w,
+// A record holds the node's region as codes (lower corner; the size level in
+// the spare top bits of x and y; in z's, whether the y and z sides are half
+// the x side; bit 31 of y set for a leaf) and, in .w,
 // the escape slot of an internal node or the cell of a leaf (whose escape is
 // always the next slot).
 __global__ __launch_bounds__(BT) void k_wparents(const int2 *__restrict__ range, const int32_t *__restrict__ split,
@@ -421,7 +422,9 @@ __global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, c
   }
   const uint32_t lx = sx ? cf.x & ~(sx - 1u) : 0u, ly = sy ? cf.y & ~(sy - 1u) : 0u, lz = sz ? cf.z & ~(sz - 1u) : 0u;
   const uint32_t w = leaf ? (uint32_t)f : (uint32_t)(pos + 2 * (l - f + 1) - 1);
-  wn[pos] = make_uint4(lx | ((L & 7u) << 29), ly | ((L >> 3) << 29) | (leaf ? 0x80000000u : 0u), lz | (rr << 29), w);
+  // z's spare bits: bit 29 set when the z side is halved, bit 30 when the y side is
+  const uint32_t halved = (rr >= 1 ? 1u : 0u) | (rr >= 2 ? 2u : 0u);
+  wn[pos] = make_uint4(lx | ((L & 7u) << 29), ly | ((L >> 3) << 29) | (leaf ? 0x80000000u : 0u), lz | (halved << 29), w);
 }
 
 __global__ void k_zero_u32(unsigned int *p, int n) {

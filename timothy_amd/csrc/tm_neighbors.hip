@@ -78,6 +78,13 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   const uint32_t mn0 = box_code(P.x - h, A.x, A.w), mx0 = box_code(P.x + h, A.x, A.w);
   const uint32_t mn1 = box_code(P.y - h, A.y, A.w), mx1 = box_code(P.y + h, A.y, A.w);
   const uint32_t mn2 = box_code(P.z - h, A.z, A.w), mx2 = box_code(P.z + h, A.z, A.w);
+  // The strict overlap of [mn, mx] with a node's [lo, lo + s) on one axis,
+  // mx > lo and mn < lo + s, is (mx - 1 - lo) < (mx - mn - 1) + s in 32-bit
+  // unsigned arithmetic: the left side wraps past every right side when
+  // mx <= lo, and otherwise the two compares agree (mx >= mn: both codes
+  // are the same monotone map of x - h and x + h).
+  const uint32_t ex0 = mx0 - 1u, ex1 = mx1 - 1u, ex2 = mx2 - 1u;
+  const uint32_t dm0 = mx0 - mn0 - 1u, dm1 = mx1 - mn1 - 1u, dm2 = mx2 - mn2 - 1u;
   // the window of slots staged in LDS: around the leaf slot of the
   // workgroup's middle cell, 2 s + 2 (e - k) for cell k of colony [s, e]
   // (up to its ancestor count), within the tree's 2 n - 1 slots
@@ -145,10 +152,11 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
       slot = visit(pq, Qp);
       pq = 0xFFFFFFFFu;
     }
-    const uint32_t L = (a.x >> 29) | (((a.y >> 29) & 3u) << 3), r = (a.z >> 29) & 3u;
+    const uint32_t L = (a.x >> 29) | (((a.y >> 29) & 3u) << 3);
     const uint32_t lx = a.x & 0x1FFFFFFFu, ly = a.y & 0x1FFFFFFFu, lz = a.z & 0x1FFFFFFFu;
-    const uint32_t sx = 1u << L, sy = sx >> (r >= 2u ? 1 : 0), sz = sx >> (r >= 1u ? 1 : 0);
-    const bool hit = (mx0 > lx) & (mn0 < lx + sx) & (mx1 > ly) & (mn1 < ly + sy) & (mx2 > lz) & (mn2 < lz + sz);
+    const uint32_t sx = 1u << L, sy = sx >> ((a.z >> 30) & 1u), sz = sx >> ((a.z >> 29) & 1u);
+    // mx > lo and mn < lo + s, as one unsigned compare per axis
+    const bool hit = (ex0 - lx < dm0 + sx) & (ex1 - ly < dm1 + sy) & (ex2 - lz < dm2 + sz);
     const bool leaf = (a.y & 0x80000000u) != 0u;
     const uint32_t next = (hit || leaf) ? node + 1 : a.w;
     if (next <= node || (leaf && a.w >= (uint32_t)n)) {  // a malformed tree: a well-formed one only moves forward
