@@ -110,17 +110,15 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   const uint32_t end = live ? (uint32_t)(2 * col_off[cid + 1] - 1) : 0u;
   uint32_t cnt = 0;
   int32_t qs = -1;
-  double4 Qs = P;  // record of the last candidate visited
   double md2 = 1.0 / 0.0;
-  // candidate q (its leaf octant overlaps the box, q != p) with its record Q:
-  // distance, mindist, the last-visited candidate, and the list entry when
+  // candidate q (its leaf octant overlaps the box, q != p) with its position
+  // Q: distance, mindist, the last-visited candidate, and the list entry when
   // within the cutoff (returned as its slot, stored by the caller; -1 if none)
   auto visit = [&](uint32_t q, const double4 &Q) -> int64_t {
     const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
     const double d2 = (dx * dx + dy * dy) + dz * dz;
     md2 = d2 < md2 ? d2 : md2;
     qs = (int32_t)q;
-    Qs = Q;
     int64_t slot = -1;
     if (d2 <= cut2) {
       if (cnt < cap) slot = tm_slot(p, cnt, cap);
@@ -195,6 +193,9 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   qstar[p] = qs;
   // v: the potential with the last candidate visited (src/potential.c:124
   // assigns it per candidate; 0 beyond the cutoff and with no candidate)
+  // the record of the last candidate, read again here rather than kept
+  // through the walk
+  const double4 Qs = qs >= 0 ? pos[qs] : P;
   vout[p] = qs >= 0 ? pot_fp64(P, young_s[p], Qs, young_s[qs], h) : 0.0;
   if (cnt > cap) atomicMax(flags + 1, cnt);
 }
