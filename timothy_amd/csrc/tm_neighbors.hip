@@ -49,9 +49,9 @@ __device__ __forceinline__ double pot_fp64(double4 P, float yp, double4 Q, float
 // pre-ordered tree, centred on the leaves of its own cells, into LDS; its
 // walks read the nodes inside that window from LDS and the rest from memory.
 // The cells of a workgroup are consecutive in Morton order, so their walks
-// share many nodes near their own leaves (over half of all visits at 2048
-// slots for 256 cells), and a read from LDS spares the texture path the
-// per-lane gather that bounds the walk.
+// share many nodes near their own leaves (about half of all visits fall in
+// 1024 slots around 256 cells), and a read from LDS spares the texture path
+// the per-lane gather that bounds the walk.
 template <int BS, int WIN>
 __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
                                                   const int32_t *__restrict__ ctype_s,
