@@ -372,39 +372,13 @@ __global__ __launch_bounds__(BT) void k_wparents(const int2 *__restrict__ range,
   par[right] = (int32_t)((k << 1) | 1);  // the child the descending walk visits first
 }
 
-__global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
-                                               const int2 *__restrict__ range, const int32_t *__restrict__ delta,
-                                               const int32_t *__restrict__ par, const int64_t *__restrict__ off,
-                                               const int32_t *__restrict__ colroot, int64_t n,
-                                               uint4 *__restrict__ wn, unsigned int *__restrict__ flags) {
-  const int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
-  if (k >= 2 * n - 1) return;
+// the record of node k (a leaf when k >= n - 1) covering cells [f, l], placed
+// at slot pos
+__device__ __forceinline__ void put_wnode(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
+                                          const int32_t *__restrict__ delta, int64_t n, int64_t k, int64_t f,
+                                          int64_t l, int64_t pos, uint4 *__restrict__ wn) {
   const bool leaf = k >= n - 1;
-  int64_t f, l;
-  if (leaf) {
-    f = l = k - (n - 1);
-  } else {
-    const int2 r = range[k];
-    f = r.x;
-    l = r.y;
-  }
   const uint4 cf = code[f];
-  const uint32_t cid = cf.w;
-  if (!leaf && code[l].w != cid) return;  // above the colonies' roots
-  const int64_t s = off[cid], e = off[cid + 1] - 1;
-  const int64_t root = colroot[cid];
-  int64_t cur = k;
-  uint32_t first = 0;
-  for (int depth = 0; cur != root; ++depth) {
-    if (depth > 256 || cur < 0 || cur >= 2 * n - 1) {  // deeper than any key allows: a malformed tree
-      atomicOr(flags, (unsigned)TM_FLAG_WALK);
-      return;
-    }
-    const int32_t p = par[cur];
-    first += (uint32_t)(p & 1);
-    cur = p >> 1;
-  }
-  const int64_t pos = 2 * s + 2 * (e - l) + first;
   uint32_t L, rr = 0, sx, sy, sz;
   if (leaf) {
     L = (uint32_t)min(span[f], 29);
@@ -425,6 +399,48 @@ __global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, c
   // z's spare bits: bit 29 set when the z side is halved, bit 30 when the y side is
   const uint32_t halved = (rr >= 1 ? 1u : 0u) | (rr >= 2 ? 2u : 0u);
   wn[pos] = make_uint4(lx | ((L & 7u) << 29), ly | ((L >> 3) << 29) | (leaf ? 0x80000000u : 0u), lz | (halved << 29), w);
+}
+
+// One thread per cell l.  The nodes whose cells end at l are the leaf l and
+// the chain above it reached through first-visited links (a parent's first
+// child holds its highest keys, so it ends where the parent ends); each
+// node is on exactly one such chain.  The thread counts the first-visited
+// links from the leaf up to the colony root once, then places the leaf and
+// its chain, each node one such link fewer than the one below it.
+__global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
+                                               const int2 *__restrict__ range, const int32_t *__restrict__ delta,
+                                               const int32_t *__restrict__ par, const int64_t *__restrict__ off,
+                                               const int32_t *__restrict__ colroot, int64_t n,
+                                               uint4 *__restrict__ wn, unsigned int *__restrict__ flags) {
+  const int64_t l = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (l >= n) return;
+  const uint32_t cid = code[l].w;
+  const int64_t s = off[cid], e = off[cid + 1] - 1;
+  const int64_t root = colroot[cid];
+  const int64_t leafk = (n - 1) + l;
+  int64_t cur = leafk;
+  uint32_t first = 0, chain = 0;
+  bool in_chain = true;
+  for (int depth = 0; cur != root; ++depth) {
+    if (depth > 256 || cur < 0 || cur >= 2 * n - 1) {  // deeper than any key allows: a malformed tree
+      atomicOr(flags, (unsigned)TM_FLAG_WALK);
+      return;
+    }
+    const int32_t p = par[cur];
+    const uint32_t fv = (uint32_t)(p & 1);
+    first += fv;
+    in_chain = in_chain && fv;
+    chain += in_chain ? 1u : 0u;
+    cur = p >> 1;
+  }
+  const int64_t base = 2 * s + 2 * (e - l);
+  put_wnode(code, span, delta, n, leafk, l, l, base + first, wn);
+  cur = leafk;
+  for (uint32_t c = 1; c <= chain; ++c) {
+    cur = par[cur] >> 1;
+    const int2 r = range[cur];
+    put_wnode(code, span, delta, n, cur, r.x, l, base + first - c, wn);
+  }
 }
 
 __global__ void k_zero_u32(unsigned int *p, int n) {
@@ -472,7 +488,7 @@ tm_status tm_run_build(tm_ctx *c) {
     k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, n, c->d_range, c->d_delta, c->d_split, c->d_nodes);
   k_colroot<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_col_off, c->ncol, c->d_range, n, c->d_colroot, c->d_flags);
   if (n > 1) k_wparents<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_range, c->d_split, n, c->d_wpar);
-  k_wnodes<<<tm_blocks(2 * n - 1, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_wpar,
+  k_wnodes<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_wpar,
                                                     c->d_col_off, c->d_colroot, n, c->d_wn, c->d_flags);
   TM_HIP(c, hipGetLastError());
   c->stage = 2;
