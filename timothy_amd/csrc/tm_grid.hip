@@ -97,12 +97,28 @@ __global__ __launch_bounds__(256) void k_deposit(const double4 *__restrict__ pos
   int cl[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) cl[a] = (int)max(min(ci[a], g.hi[a] + 1), g.lo[a] - 1);
-  __syncthreads();
-  if (dep) {
+  // the box over the depositing lanes: reduced across the wave first, then
+  // one LDS atomic per wave and bound (not one per lane on the same word)
+  int bmin[3], bmax[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    bmin[a] = dep ? cl[a] : 0x7FFFFFFF;
+    bmax[a] = dep ? cl[a] : -0x7FFFFFFF;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&box[a], cl[a]);
-      atomicMax(&box[3 + a], cl[a]);
+      bmin[a] = min(bmin[a], __shfl_xor(bmin[a], o, 64));
+      bmax[a] = max(bmax[a], __shfl_xor(bmax[a], o, 64));
+    }
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&box[a], bmin[a]);
+      atomicMax(&box[3 + a], bmax[a]);
     }
   }
   __syncthreads();
