@@ -31,6 +31,64 @@ constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 20; }  // 12 bits
 
+// Ascending bitonic sort of list[0, m) (m <= 256 E) by the 256 threads of
+// the workgroup, each holding E consecutive elements (padding EMPTY sorts
+// last).  A compare-exchange partner at distance j lies in the same thread
+// (j < E), in another lane of the same wave (j < 64 E: a shuffle), or in
+// another wave (through LDS, the only stages that need the workgroup's
+// barrier).  Element i keeps the smaller of itself and i ^ j when its place
+// in the pair (i & j) matches the direction of its run (i & size).
+__device__ __forceinline__ uint32_t bitonic_pick(uint32_t x, uint32_t y, uint32_t i, uint32_t j, uint32_t size) {
+  const bool lower = (i & j) == 0, asc = (i & size) == 0;
+  return lower == asc ? min(x, y) : max(x, y);
+}
+
+template <int E>
+__device__ void tile_sort(uint32_t *list, uint32_t m, uint32_t t) {
+  constexpr uint32_t N = (uint32_t)TM_TILE * E;
+  uint32_t a[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t i = t * E + e;
+    a[e] = i < m ? list[i] : 0xFFFFFFFFu;
+  }
+  for (uint32_t size = 2; size <= N; size <<= 1) {
+    for (uint32_t j = size >> 1; j > 0; j >>= 1) {
+      if (j >= 64u * E) {
+        __syncthreads();  // the previous stage's readers are done
+#pragma unroll
+        for (int e = 0; e < E; ++e) list[t * E + e] = a[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint32_t i = t * E + e;
+          a[e] = bitonic_pick(a[e], list[i ^ j], i, j, size);
+        }
+      } else if (j >= (uint32_t)E) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const uint32_t i = t * E + e;
+          const uint32_t y = (uint32_t)__shfl_xor((int)a[e], (int)(j / E), 64);
+          a[e] = bitonic_pick(a[e], y, i, j, size);
+        }
+      } else {
+        uint32_t b[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) b[e] = a[e ^ j];
+#pragma unroll
+        for (int e = 0; e < E; ++e) a[e] = bitonic_pick(a[e], b[e], t * E + e, j, size);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t i = t * E + e;
+    if (i < m) list[i] = a[e];
+  }
+  __syncthreads();
+}
+
 // exclusive prefix sum over the 256 threads of the workgroup
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *wsum) {
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -122,23 +180,11 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     const uint32_t k = keys[t * (HASH / TS) + i];
     if (k != EMPTY) list[w++] = k;
   }
-  uint32_t p2 = 64;
-  while (p2 < m) p2 <<= 1;
-  for (uint32_t i = m + t; i < p2; i += TS) list[i] = EMPTY;
   __syncthreads();
-  for (uint32_t size = 2; size <= p2; size <<= 1) {
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      for (uint32_t i = t; i < p2 / 2; i += TS) {
-        const uint32_t lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
-        const uint32_t a = list[lo], c = list[hi];
-        if ((a > c) == ((lo & size) == 0)) {
-          list[lo] = c;
-          list[hi] = a;
-        }
-      }
-      __syncthreads();
-    }
-  }
+  static_assert(TS == 256 && 8 * TS >= TM_TILE_MAX && SORTN >= 8 * TS, "tile_sort sizes");
+  if (m <= 2 * TS) tile_sort<2>(list, m, t);
+  else if (m <= 4 * TS) tile_sort<4>(list, m, t);
+  else tile_sort<8>(list, m, t);
   // local position of every distinct neighbour
   for (uint32_t i = t; i < m; i += TS) {
     const uint32_t q = list[i];
