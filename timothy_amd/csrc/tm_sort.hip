@@ -57,6 +57,39 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
   hist[(int64_t)tid * nblocks + blockIdx.x] = s;
 }
 
+// Exclusive scan of each digit's row of per-tile counts (one workgroup per
+// digit), the row's total stored after the rows: the scatter adds the
+// totals of the smaller digits itself.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_rowscan(uint32_t *__restrict__ hist, int nblocks) {
+  __shared__ uint32_t wsum[RS_WAVES];
+  const int d = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t *row = hist + (int64_t)d * nblocks;
+  const int per = (nblocks + RS_THREADS - 1) / RS_THREADS;
+  const int b0 = tid * per, b1 = min(b0 + per, nblocks);
+  uint32_t sum = 0;
+  for (int b = b0; b < b1; ++b) sum += row[b];
+  uint32_t inc = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t run = inc - sum, all = 0;
+#pragma unroll
+  for (int q = 0; q < RS_WAVES; ++q) {
+    if (q < w) run += wsum[q];
+    all += wsum[q];
+  }
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t v = row[b];
+    row[b] = run;
+    run += v;
+  }
+  if (tid == 0) hist[(int64_t)256 * nblocks + d] = all;
+}
+
 // The tile's keys are ranked stably (by digit, then input order), placed in
 // LDS in that order, and written out from there: consecutive lanes write
 // consecutive slots of one digit's run instead of 64 scattered records.
@@ -74,7 +107,26 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__res
   __shared__ int32_t sval[RS_TILE];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int j = tid; j < RS_WAVES * 256; j += RS_THREADS) (&cnt[0][0])[j] = 0;
-  gbase[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+  // this tile's first slot for digit tid: its row prefix (offs, per digit
+  // over the tiles) plus the keys of all smaller digits (the row totals
+  // after the rows, scanned here)
+  {
+    const uint32_t rt = offs[(int64_t)256 * nblocks + tid];
+    uint32_t inc = rt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t wb = 0;
+#pragma unroll
+    for (int q = 0; q < RS_WAVES; ++q)
+      if (q < w) wb += wsum[q];
+    gbase[tid] = offs[(int64_t)tid * nblocks + blockIdx.x] + wb + inc - rt;
+    __syncthreads();
+  }
   const int64_t tile0 = (int64_t)blockIdx.x * RS_TILE;
   const int64_t base = tile0 + (int64_t)w * (64 * RS_ITEMS);
   uint64_t k[RS_ITEMS];
@@ -251,13 +303,14 @@ tm_status tm_scan_u32(tm_ctx *c, const uint32_t *in, uint32_t *out, int64_t n) {
 tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n) {
   if (n <= 1) return TM_OK;
   int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
-  TM_TRY(tm_ensure(c, &c->d_hist, c->cap_hist, (int64_t)256 * nblocks, "radix histogram"));
+  TM_TRY(tm_ensure(c, &c->d_hist, c->cap_hist, (int64_t)256 * nblocks + 256, "radix histogram"));
   uint64_t *k0 = keys, *k1 = c->d_key_alt;
   int32_t *v0 = vals, *v1 = c->d_val_alt;
   for (int shift = 0; shift < 64; shift += 8) {
     k_rs_hist<<<nblocks, RS_THREADS, 0, c->stream>>>(k0, n, shift, c->d_hist, nblocks);
     TM_HIP(c, hipGetLastError());
-    TM_TRY(tm_scan_u32(c, c->d_hist, c->d_hist, (int64_t)256 * nblocks));
+    k_rs_rowscan<<<256, RS_THREADS, 0, c->stream>>>(c->d_hist, nblocks);
+    TM_HIP(c, hipGetLastError());
     k_rs_scatter<<<nblocks, RS_THREADS, 0, c->stream>>>(k0, v0, k1, v1, n, shift, c->d_hist, nblocks);
     TM_HIP(c, hipGetLastError());
     uint64_t *tk = k0;
