@@ -104,16 +104,18 @@ def test_launch_shapes_invariant(tm, oracle_mod, ctx):
     exp_q = np.array([cand[coff[p + 1] - 1] if coff[p + 1] > coff[p] else -1 for p in range(col["n"])])
     ctx.neighbors()
     lists = ctx.neighbor_lists()
-    for nb, win in ((64, 0), (128, 0), (256, 0), (256, 512), (256, 2048), (256, 1024)):
-        say(f"nbr_block {nb} walk_window {win}")
-        ctx.set_param("nbr_block", nb)
-        ctx.set_param("walk_window", win)
-        ctx.neighbors()
-        got = ctx.neighbor_lists()
-        assert np.array_equal(got[0], lists[0]) and np.array_equal(got[1], lists[1])
-        assert np.array_equal(ctx.qstar(), exp_q)
-        ctx.sph(tm.FP64)
-        assert np.array_equal(ctx.results()["mindist"], ref["mindist"])
+    for f32 in (0, 1):
+        for nb, win in ((64, 0), (128, 0), (256, 0), (256, 512), (256, 2048), (256, 1024)):
+            say(f"nbr_block {nb} walk_window {win} walk_f32 {f32}")
+            ctx.set_param("walk_f32", f32)
+            ctx.set_param("nbr_block", nb)
+            ctx.set_param("walk_window", win)
+            ctx.neighbors()
+            got = ctx.neighbor_lists()
+            assert np.array_equal(got[0], lists[0]) and np.array_equal(got[1], lists[1])
+            assert np.array_equal(ctx.qstar(), exp_q)
+            ctx.sph(tm.FP64)
+            assert np.array_equal(ctx.results()["mindist"], ref["mindist"])
     for tiles in (0, 1):
         ctx.set_param("sph_tiles", tiles)
         ctx.neighbors()
@@ -460,6 +462,60 @@ def test_colony_far_from_origin(tm, oracle_mod, ctx):
     for p in range(col["n"]):
         c = cand[coff[p]:coff[p + 1]]
         assert np.array_equal(nbr[off[p]:off[p + 1]], c[inh[coff[p]:coff[p + 1]] == 1]), p
+
+
+def _lattice(m, s, shift):
+    """m^3 cells on a cubic lattice of spacing 2 s (= the cutoff of type 0
+    scaled by s): six neighbours of each inner cell sit exactly at the cutoff
+    and tie for the minimum distance"""
+    g = np.arange(m, dtype=np.float64) * 2.0
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    n = m ** 3
+    types = [dict(size=t["size"] * s, h=t["h"] * s, h2=t["h2"] * s ** 2, h3=t["h3"] * s ** 3, h4=t["h4"] * s ** 4)
+             for t in gen.CELLTYPES]
+    young = (2100.0 + 7.0 * (np.arange(n) % 13)).astype(np.float32)
+    return dict(n=n, x=X.ravel() * s + shift, y=Y.ravel() * s - 0.5 * shift, z=Z.ravel() * s + 0.25 * shift,
+                size=np.full(n, types[0]["size"]), young=young, ctype=np.zeros(n, np.int32),
+                phase=np.ones(n, np.int32), types=types)
+
+
+def test_walk_fp32_screening_matches_fp64_walk(tm, oracle_mod, ctx):
+    """the walk's FP32 candidate screening (walk_f32, on by default) leaves
+    every output of the FP64 walk unchanged: lists, qstar, mindist, v and the
+    FP64 sums, on lattices whose pairs sit at the cutoff and tie for the
+    minimum, at tiny and huge scales far from the origin, and on a Gaussian
+    colony shifted to 1e6"""
+    O = oracle_mod
+    far = dict(CASES[1][1]())
+    for k, d in (("x", 1.0e6), ("y", -2.5e6), ("z", 3.0e5)):
+        far[k] = np.asarray(far[k]) + d
+    cases = [("lattice", _lattice(12, 1.0, 0.0)), ("lattice_small", _lattice(10, 1e-5, 3e-2)),
+             ("lattice_huge", _lattice(10, 1e7, 1e12)), ("gauss_far", far)]
+    try:
+        for name, col in cases:
+            ctx.upload_colony(col)
+            out = {}
+            for f in (0, 1):
+                say(f"{name} n={col['n']} walk_f32 {f}")
+                ctx.set_param("walk_f32", f)
+                ctx.build()
+                ctx.neighbors()
+                ctx.sph(tm.FP64)
+                out[f] = (ctx.neighbor_lists(), ctx.qstar(), ctx.results())
+            (l0, q0, r0), (l1, q1, r1) = out[0], out[1]
+            assert np.array_equal(l0[0], l1[0]) and np.array_equal(l0[1], l1[1]), name
+            assert np.array_equal(q0, q1), name
+            for k in ("density", "v", "mindist", "force"):
+                assert np.array_equal(r0[k], r1[k]), (name, k)
+            if name == "lattice":
+                ref = oracle_results(O.Octree(O.Colony.from_dict(col), col["types"]))
+                for k in ("density", "v", "mindist", "force"):
+                    assert np.array_equal(r1[k], ref[k]), k
+                # inner cells: six neighbours at exactly the cutoff
+                cnt = np.diff(l1[0])
+                assert cnt.max() == 6 and np.all(r1["mindist"] == 2.0)
+    finally:
+        ctx.set_param("walk_f32", 1)
 
 
 def test_graph_replay_matches_eager(tm, ctx):

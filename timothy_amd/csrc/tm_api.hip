@@ -36,6 +36,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
   free_ptr(c->d_perm);
   free_ptr(c->d_code);
   free_ptr(c->d_pos);
+  free_ptr(c->d_posf);
   free_ptr(c->d_young_s);
   free_ptr(c->d_ctype_s);
   free_ptr(c->d_phase_s);
@@ -67,7 +68,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
     size_t bytes;
   } list[] = {
       {(void **)&c->d_key, 8},      {(void **)&c->d_key_alt, 8},  {(void **)&c->d_val_alt, 4},
-      {(void **)&c->d_perm, 4},     {(void **)&c->d_code, 16},    {(void **)&c->d_pos, 32},
+      {(void **)&c->d_perm, 4},     {(void **)&c->d_code, 16},    {(void **)&c->d_pos, 32},    {(void **)&c->d_posf, 16},
       {(void **)&c->d_young_s, 4},  {(void **)&c->d_ctype_s, 4},
       {(void **)&c->d_phase_s, 4},  {(void **)&c->d_span, 4},     {(void **)&c->d_nodes, 64},
       {(void **)&c->d_wn, 32},      {(void **)&c->d_wpar, 8},     {(void **)&c->d_range, 8},    {(void **)&c->d_delta, 4},    {(void **)&c->d_ncount, 4},
@@ -289,6 +290,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_perm);
   free_ptr(c->d_code);
   free_ptr(c->d_pos);
+  free_ptr(c->d_posf);
   free_ptr(c->d_young_s);
   free_ptr(c->d_ctype_s);
   free_ptr(c->d_phase_s);
@@ -415,6 +417,8 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->sph_tiles = v ? 1 : 0;
   } else if (!strcmp(key, "tile_cap4")) {
     c->tile_cap4 = v ? 1 : 0;
+  } else if (!strcmp(key, "walk_f32")) {
+    c->walk_f32 = v ? 1 : 0;
   } else if (!strcmp(key, "walk_window")) {
     if (v != 0 && v != 512 && v != 1024 && v != 2048)
       return tm_fail(c, TM_ERR_INVALID, "walk_window in {0,512,1024,2048}");
@@ -437,6 +441,7 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "sph_tiles")) *v = c->sph_tiles;
   else if (!strcmp(key, "tile_cap4")) *v = c->tile_cap4;
   else if (!strcmp(key, "walk_window")) *v = c->walk_window;
+  else if (!strcmp(key, "walk_f32")) *v = c->walk_f32;
   else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
   else return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
@@ -882,6 +887,18 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   }
   c->nbr_block = best_nb;
   c->walk_window = best_win;
+  // that shape with the FP32 candidate screening on and off (same results)
+  {
+    const int f0 = c->walk_f32;
+    double t[2] = {0, 0};
+    for (int k = 0; k < 2; ++k) {
+      c->walk_f32 = k == 0 ? f0 : !f0;
+      TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 10, &t[k]));
+      snprintf(line, sizeof(line), "walk_f32=%d %.4f ms\n", c->walk_f32, t[k]);
+      rep += line;
+    }
+    c->walk_f32 = t[1] < t[0] ? !f0 : f0;
+  }
   TM_TRY(tm_run_neighbors(c, true));
   // pair kernel: workgroup size x neighbours fetched per trip
   int best_sb = c->sph_block;
@@ -911,8 +928,9 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   c->sph_block = best_sb;
   unroll = best_u;
   if (p == TM_MIXED) c->sph_lanes = best_l;
-  snprintf(line, sizeof(line), "chosen nbr_block=%d sph_block=%d unroll=%d lanes=%d (%s)\n", best_nb, best_sb,
-           best_u, p == TM_MIXED ? best_l : 1, p == TM_MIXED ? "mixed" : "fp64");
+  snprintf(line, sizeof(line), "chosen nbr_block=%d walk_window=%d walk_f32=%d sph_block=%d unroll=%d lanes=%d (%s)\n",
+           best_nb, best_win, c->walk_f32, best_sb, best_u, p == TM_MIXED ? best_l : 1,
+           p == TM_MIXED ? "mixed" : "fp64");
   rep += line;
   if (buf && buflen) {
     strncpy(buf, rep.c_str(), buflen - 1);

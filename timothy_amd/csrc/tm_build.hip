@@ -443,6 +443,18 @@ __global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, c
   }
 }
 
+// x, y, z of each sorted cell relative to its colony's origin (the lower
+// corner of the bounding box), formed in FP64 and rounded to FP32: the
+// walk's screening copy of the positions
+__global__ __launch_bounds__(BT) void k_posf(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
+                                             const double4 *__restrict__ aff, int64_t n, float4 *__restrict__ posf) {
+  const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (i >= n) return;
+  const double4 P = pos[i];
+  const double4 A = aff[code[i].w];
+  posf[i] = make_float4((float)(P.x - A.x), (float)(P.y - A.y), (float)(P.z - A.z), 0.0f);
+}
+
 __global__ void k_zero_u32(unsigned int *p, int n) {
   int i = threadIdx.x;
   if (i < n) p[i] = 0;
@@ -481,6 +493,7 @@ tm_status tm_run_build(tm_ctx *c) {
                                              c->d_young_s, c->d_ctype_s, c->d_phase_s);
   k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_young_s,
                                               c->d_ctype_s, c->d_phase_s, c->d_flags);
+  k_posf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_pos, c->d_code, c->d_aff, n, c->d_posf);
   k_leaf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, n, c->d_span, c->d_flags);
   TM_HIP(c, hipGetLastError());
   // radix tree

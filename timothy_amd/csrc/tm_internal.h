@@ -76,6 +76,7 @@ struct tm_ctx {
   int sph_unroll_mixed = 4;  // the same for the mixed kernel
   int nbr_block = 256;
   int walk_window = 1024;  // slots of the tree a 256-cell walk workgroup stages in LDS (0, 512, 1024, 2048)
+  int walk_f32 = 1;  // the walk screens candidates with FP32 positions and an error bound (results unchanged)
   int sph_lanes = 1;
   int sph_balance = 0;  // FP64 kernel without tiles: rank cells by list length inside each workgroup
 
@@ -116,6 +117,7 @@ struct tm_ctx {
   int32_t *d_perm = nullptr;
   uint4 *d_code = nullptr;  // x, y, z codes, colony id
   double4 *d_pos = nullptr; // x, y, z, size
+  float4 *d_posf = nullptr;  // x, y, z relative to the colony's origin, rounded to FP32 (the walk's candidate reads)
   float *d_young_s = nullptr;
   int32_t *d_ctype_s = nullptr, *d_phase_s = nullptr;
   int32_t *d_span = nullptr;

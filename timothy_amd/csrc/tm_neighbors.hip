@@ -18,6 +18,8 @@
 //   v       that potential, in the reference's FP64 expressions
 // The cutoff test d <= h is evaluated exactly as d2 <= cut2, cut2 being the
 // largest double whose correctly rounded root is <= h.
+#include <type_traits>
+
 #include "tm_internal.h"
 
 namespace {
@@ -52,8 +54,41 @@ __device__ __forceinline__ double pot_fp64(double4 P, float yp, double4 Q, float
 // share many nodes near their own leaves (about half of all visits fall in
 // 1024 slots around 256 cells), and a read from LDS spares the texture path
 // the per-lane gather that bounds the walk.
-template <int BS, int WIN>
-__global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
+// reference squared distance, src/potential.c:49 order (contraction is off)
+__device__ __forceinline__ double dist2_fp64(const double4 &P, const double4 &Q) {
+  const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+// F32: candidates are screened with their FP32 positions relative to the
+// colony's origin (16 bytes per candidate instead of 32); only a candidate
+// whose FP32 distance cannot decide the cutoff, or that may hold the minimum
+// distance, has its FP64 position read.  The results are those of the FP64
+// walk (the bounds below are proved, not tuned):
+//   stored values  |p~ - (P - A)| <= W (2^-24 + 2^-53) + 2^-150 per axis,
+//                  W = the colony's largest extent >= |P - A|, so the FP32
+//                  difference vector is within K = sqrt(3) * 1.2e-7 W
+//                  (+1e-44) of the exact one after its own 2^-24 rounding,
+//                  which is taken relative (1e-7) to the FP32 distance
+//   FP32 sum       s~ = fma(dz, dz, fma(dy, dy, dx dx)): relative 3e-7,
+//                  absolute 1e-44 from underflow
+//   FP64 sum       the reference's d2 is within 1e-15 (relative) of the
+//                  exact squared distance
+// so with r~ = sqrt(s~), the reference's d2 lies in
+//   [((r~ (1 - 1e-7) - K)+)^2 (1 - 1e-15), (r~ (1 + 1e-7) + K)^2 (1 + 1e-15)]
+// (r~ widened by 3e-7 and 1e-44 as above).  tin / tout are the FP32 values
+// of s~ below which d2 <= cut2 holds and above which it fails, each rounded
+// inward by 1e-6; between them the candidate is read in FP64.  For the
+// minimum, a candidate c cannot hold it when s~_c > 1.0002 s~_b + beta for
+// a candidate b already seen: the bound above on both sides, with the cross
+// term 4.02 K r~ <= 1e-4 r~^2 + 40401 K^2 (AM-GM), gives that threshold.
+// The smallest FP32 distance is kept with its cell; the candidate that loses
+// that place (the newcomer, or the one it displaces) is read in FP64 at once
+// when it is still within reach of the smallest (rare), and the smallest is
+// read in FP64 at the end.
+template <int BS, int WIN, bool F32>
+__global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const float4 *__restrict__ posf,
+                                                  const uint4 *__restrict__ code,
                                                   const int32_t *__restrict__ ctype_s,
                                                   const float *__restrict__ young_s,
                                                   const uint4 *__restrict__ wn,
@@ -114,13 +149,61 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   // candidate q (its leaf octant overlaps the box, q != p) with its position
   // Q: distance, mindist, the last-visited candidate, and the list entry when
   // within the cutoff (returned as its slot, stored by the caller; -1 if none)
-  auto visit = [&](uint32_t q, const double4 &Q) -> int64_t {
-    const double dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-    const double d2 = (dx * dx + dy * dy) + dz * dz;
+  using QRec = typename std::conditional<F32, float4, double4>::type;
+  // FP32 screening state: the cell's FP32 relative position, the cutoff
+  // thresholds, and the smallest FP32 distance so far with its cell
+  float pfx = 0.f, pfy = 0.f, pfz = 0.f, tin = -1.f, tout = 0.f, beta = 0.f;
+  float s1 = 1.0f / 0.0f;
+  uint32_t c1 = 0xFFFFFFFFu;
+  if constexpr (F32) {
+    pfx = (float)(P.x - A.x);
+    pfy = (float)(P.y - A.y);
+    pfz = (float)(P.z - A.z);
+    const double K = 1.7321 * (A.w * 1.2e-7 + 1e-44);
+    const double hc = sqrt(cut2);
+    double ti = (hc * (1.0 - 1e-15) - K) / (1.0 + 1e-7);
+    ti = ti > 0.0 ? ti * ti / (1.0 + 3e-7) - 1e-44 : -1.0;
+    // a threshold outside FP32's normal range decides nothing
+    tin = (ti >= 1e-30 && ti <= 1e38) ? (float)(ti * (1.0 - 1e-6)) : -1.0f;
+    double to = (hc * (1.0 + 1e-15) + K) / (1.0 - 1e-7);
+    to = to * to * (1.0 + 3e-7) + 1e-44;
+    // (overflow gives inf and NaN stays NaN: both decide nothing; below
+    // FP32's normal range the threshold is raised, which only widens the
+    // band read in FP64)
+    tout = to < 1e-30 ? 1e-30f : (float)(to * (1.0 + 1e-6));
+    beta = (float)((40500.0 * K * K + 1e-37) * (1.0 + 1e-6));
+  }
+  // the reference's d2 of candidate q, read in FP64, into the minimum
+  auto exact = [&](uint32_t q) -> double {
+    const double d2 = dist2_fp64(P, pos[q]);
     md2 = d2 < md2 ? d2 : md2;
+    return d2;
+  };
+  auto visit = [&](uint32_t q, const QRec &Q) -> int64_t {
+    bool in;
+    if constexpr (F32) {
+      const float dx = pfx - Q.x, dy = pfy - Q.y, dz = pfz - Q.z;
+      const float sq = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      if (sq <= tin) in = true;
+      else if (sq > tout) in = false;
+      else in = exact(q) <= cut2;
+      // the smallest (selects); the one that loses it
+      const bool lt = sq < s1;
+      const float so = lt ? s1 : sq;
+      const uint32_t co = lt ? c1 : q;
+      s1 = lt ? sq : s1;
+      c1 = lt ? q : c1;
+      if (so <= fmaf(1.0002f, s1, beta)) {
+        if (co != 0xFFFFFFFFu) (void)exact(co);
+      }
+    } else {
+      const double d2 = dist2_fp64(P, Q);
+      md2 = d2 < md2 ? d2 : md2;
+      in = d2 <= cut2;
+    }
     qs = (int32_t)q;
     int64_t slot = -1;
-    if (d2 <= cut2) {
+    if (in) {
       if (cnt < cap) slot = tm_slot(p, cnt, cap);
       ++cnt;
     }
@@ -135,7 +218,7 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   // memory counters retire in order).  Candidates are still taken in visit
   // order.
   uint32_t pq = 0xFFFFFFFFu;  // the candidate whose record is in flight
-  double4 Qp = P;
+  QRec Qp{};
   while (node < end) {
     uint4 a;
     if constexpr (WIN > 0) {
@@ -163,7 +246,8 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
       break;
     }
     if (leaf && hit && a.w != (uint32_t)p) {
-      Qp = pos[a.w];
+      if constexpr (F32) Qp = posf[a.w];
+      else Qp = pos[a.w];
       pq = a.w;
     }
     if (slot >= 0) nbr[slot] = q;
@@ -188,6 +272,10 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   for (uint32_t k = lo; k < g; ++k)
     if (live && k >= used) nbr[tm_slot(p, k, cap)] = (uint32_t)n;
   if (!live) return;
+  if constexpr (F32) {
+    // the smallest
+    if (c1 != 0xFFFFFFFFu) (void)exact(c1);
+  }
   count[p] = cnt;
   mindist[p] = qs >= 0 ? sqrt(md2) : 1.7976931348623157e308;
   qstar[p] = qs;
@@ -202,8 +290,9 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
 
 template <int BS, int WIN>
 void launch_walk(tm_ctx *c) {
-  k_neighbors<BS, WIN><<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
-      c->d_pos, c->d_code, c->d_ctype_s, c->d_young_s, c->d_wn, c->d_col_off, c->d_aff, c->d_types, c->n,
+  auto kern = c->walk_f32 ? k_neighbors<BS, WIN, true> : k_neighbors<BS, WIN, false>;
+  kern<<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
+      c->d_pos, c->d_posf, c->d_code, c->d_ctype_s, c->d_young_s, c->d_wn, c->d_col_off, c->d_aff, c->d_types, c->n,
       c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_v, c->d_flags);
 }
 
