@@ -518,6 +518,47 @@ def test_walk_fp32_screening_matches_fp64_walk(tm, oracle_mod, ctx):
         ctx.set_param("walk_f32", 1)
 
 
+def test_key_prefix_ties(tm, oracle_mod, ctx):
+    """cells whose codes differ only below bit 8 (their 64-bit sort keys tie
+    and the build orders them by the full key): lists and FP64 sums match the
+    oracle bit for bit, with the FP32 screening on and off"""
+    O = oracle_mod
+    col = dict(gen.config(1, scale=0.002))
+    n0 = col["n"]
+    twins = np.arange(0, n0, max(1, n0 // 60))[:60]
+    ext = max(np.ptp(col["x"]), np.ptp(col["y"]), np.ptp(col["z"]))
+    d = ext * 2.0 ** -22  # about 2^7 code units on each axis
+    extra = dict(x=col["x"][twins] + d, y=col["y"][twins] - d, z=col["z"][twins] + 0.5 * d)
+    for k in ("x", "y", "z"):
+        col[k] = np.concatenate([col[k], extra[k]])
+    for k in ("size", "young", "ctype", "phase"):
+        col[k] = np.concatenate([col[k], np.asarray(col[k])[twins]])
+    col["n"] = n0 + len(twins)
+    col.pop("grid", None)
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    ref = oracle_results(tree)
+    key_hi = None
+    try:
+        for f in (1, 0):
+            say(f"key prefix ties n={col['n']} walk_f32 {f}")
+            ctx.set_param("walk_f32", f)
+            ctx.upload_colony(col)
+            ctx.build()
+            loc = ctx.codes()
+            if key_hi is None:
+                # the twins do tie in their top 21 levels
+                hi = [tuple(int(v) >> 8 for v in row) for row in loc]
+                key_hi = len(hi) - len(set(hi))
+                assert key_hi > 0
+            ctx.neighbors()
+            ctx.sph(tm.FP64)
+            r = ctx.results()
+            for k in ("density", "v", "mindist", "force"):
+                assert np.array_equal(r[k], ref[k]), (f, k)
+    finally:
+        ctx.set_param("walk_f32", 1)
+
+
 def test_graph_replay_matches_eager(tm, ctx):
     col = gen.config(1, scale=0.02)
     ctx.upload_colony(col)

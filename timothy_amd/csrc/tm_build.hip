@@ -184,8 +184,9 @@ __global__ __launch_bounds__(BT) void k_keys(const double *__restrict__ x, const
 __global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm, int64_t n,
                                                const double4 *__restrict__ pos_o, const uint4 *__restrict__ aux_o,
                                                const double4 *__restrict__ aff, uint4 *__restrict__ code,
-                                               double4 *__restrict__ pos, float *__restrict__ young_s,
-                                               int32_t *__restrict__ ctype_s, int32_t *__restrict__ phase_s) {
+                                               double4 *__restrict__ pos, float4 *__restrict__ posf,
+                                               float *__restrict__ young_s, int32_t *__restrict__ ctype_s,
+                                               int32_t *__restrict__ phase_s) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i == 0) pos[n] = make_double4(TM_FAR, TM_FAR, TM_FAR, 0.0);  // stand-in neighbour padding the lists
   if (i >= n) return;
@@ -196,6 +197,9 @@ __global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm,
   const double4 a = aff[X.w];
   code[i] = make_uint4(loc_code(P.x, a.x, a.w), loc_code(P.y, a.y, a.w), loc_code(P.z, a.z, a.w), X.w);
   pos[i] = P;
+  // the walk's screening copy: x, y, z relative to the colony's lower
+  // corner, formed in FP64 and rounded to FP32
+  posf[i] = make_float4((float)(P.x - a.x), (float)(P.y - a.y), (float)(P.z - a.z), 0.0f);
   young_s[i] = __uint_as_float(X.x);
   ctype_s[i] = (int32_t)X.y;
   phase_s[i] = (int32_t)X.z;
@@ -204,8 +208,8 @@ __global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm,
 // Cells whose 64-bit key prefixes tie are put in full 87-bit key order; the
 // first thread of each run sorts it (runs are a few cells at most).
 __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key, int64_t n, int32_t *perm, uint4 *code,
-                                                double4 *pos, float *young_s, int32_t *ctype_s, int32_t *phase_s,
-                                                unsigned int *flags) {
+                                                double4 *pos, float4 *posf, float *young_s, int32_t *ctype_s,
+                                                int32_t *phase_s, unsigned int *flags) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n - 1) return;
   if (key[i + 1] != key[i]) return;
@@ -220,6 +224,7 @@ __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key
     uint4 c = code[a];
     int32_t pm = perm[a];
     double4 ps = pos[a];
+    float4 pf = posf[a];
     float yg = young_s[a];
     int32_t ct = ctype_s[a], ph = phase_s[a];
     int64_t b = a - 1;
@@ -227,6 +232,7 @@ __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key
       code[b + 1] = code[b];
       perm[b + 1] = perm[b];
       pos[b + 1] = pos[b];
+      posf[b + 1] = posf[b];
       young_s[b + 1] = young_s[b];
       ctype_s[b + 1] = ctype_s[b];
       phase_s[b + 1] = phase_s[b];
@@ -235,6 +241,7 @@ __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key
     code[b + 1] = c;
     perm[b + 1] = pm;
     pos[b + 1] = ps;
+    posf[b + 1] = pf;
     young_s[b + 1] = yg;
     ctype_s[b + 1] = ct;
     phase_s[b + 1] = ph;
@@ -443,18 +450,6 @@ __global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, c
   }
 }
 
-// x, y, z of each sorted cell relative to its colony's origin (the lower
-// corner of the bounding box), formed in FP64 and rounded to FP32: the
-// walk's screening copy of the positions
-__global__ __launch_bounds__(BT) void k_posf(const double4 *__restrict__ pos, const uint4 *__restrict__ code,
-                                             const double4 *__restrict__ aff, int64_t n, float4 *__restrict__ posf) {
-  const int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
-  if (i >= n) return;
-  const double4 P = pos[i];
-  const double4 A = aff[code[i].w];
-  posf[i] = make_float4((float)(P.x - A.x), (float)(P.y - A.y), (float)(P.z - A.z), 0.0f);
-}
-
 __global__ void k_zero_u32(unsigned int *p, int n) {
   int i = threadIdx.x;
   if (i < n) p[i] = 0;
@@ -490,10 +485,9 @@ tm_status tm_run_build(tm_ctx *c) {
   TM_HIP(c, hipGetLastError());
   TM_TRY(tm_sort_pairs(c, c->d_key, c->d_perm, n));
   k_gather<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_perm, n, c->d_pos_o, c->d_aux_o, c->d_aff, c->d_code, c->d_pos,
-                                             c->d_young_s, c->d_ctype_s, c->d_phase_s);
-  k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_young_s,
-                                              c->d_ctype_s, c->d_phase_s, c->d_flags);
-  k_posf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_pos, c->d_code, c->d_aff, n, c->d_posf);
+                                             c->d_posf, c->d_young_s, c->d_ctype_s, c->d_phase_s);
+  k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_posf,
+                                              c->d_young_s, c->d_ctype_s, c->d_phase_s, c->d_flags);
   k_leaf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, n, c->d_span, c->d_flags);
   TM_HIP(c, hipGetLastError());
   // radix tree
