@@ -27,6 +27,7 @@ namespace {
 constexpr int TS = TM_TILE;
 constexpr int HASH = 4096;  // open-addressing set of distinct neighbours (> 2 TM_TILE_MAX)
 constexpr int SORTN = 2048;  // power of two >= TM_TILE_MAX
+constexpr int WB = 16384;    // cells around a tile's middle whose order a bitmap gives
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 20; }  // 12 bits
@@ -117,14 +118,16 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   __shared__ uint32_t rcnt[TS];
   __shared__ uint8_t order[TS];
   __shared__ uint32_t wsum[TS / 64];
-  __shared__ uint32_t distinct;
+  __shared__ uint32_t bm[WB / 32], pre[WB / 32];
+  __shared__ uint32_t distinct, nout, nbelow;
   const int64_t g = blockIdx.x;
   const uint32_t t = threadIdx.x;
   const int64_t cell = g * TS + t;
   const uint32_t cnt = cell < n ? min(count[cell], (uint32_t)cap) : 0u;
   for (int i = t; i < HASH; i += TS) keys[i] = EMPTY;
+  for (int i = t; i < WB / 32; i += TS) bm[i] = 0;
   bins[t] = 0;
-  if (t == 0) distinct = 0;
+  if (t == 0) distinct = nout = nbelow = 0;
   __syncthreads();
 
   // rank the cells by list length, longest first (counting sort; the order
@@ -172,26 +175,72 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     }
     return;
   }
-  // compact the set in slot order, then sort ascending
-  uint32_t mine = 0;
-  for (int i = 0; i < HASH / TS; ++i) mine += keys[t * (HASH / TS) + i] != EMPTY;
-  uint32_t w = block_scan(mine, wsum);
+  // Order the set ascending.  Most of a tile's neighbours lie within WB / 2
+  // cells of its middle (84% in the 2M-cell Gaussian colony): those are
+  // marked in a bitmap, which holds them in order; only the others are
+  // compacted and sorted.  The order is then: the sorted ones below the
+  // window, the window's, the sorted ones above it.  A set of at most TS
+  // is sorted whole (one element per thread).
+  const bool use_bm = m > (uint32_t)TS;
+  const int64_t wlo = g * TS + TS / 2 - WB / 2;  // may be negative
+  uint32_t mine = 0, low = 0;
   for (int i = 0; i < HASH / TS; ++i) {
     const uint32_t k = keys[t * (HASH / TS) + i];
-    if (k != EMPTY) list[w++] = k;
+    if (k == EMPTY) continue;
+    const int64_t r = (int64_t)k - wlo;
+    if (use_bm && r >= 0 && r < WB) {
+      atomicOr(&bm[r >> 5], 1u << (r & 31));
+    } else {
+      ++mine;
+      low += r < 0;
+    }
+  }
+  if (mine) atomicAdd(&nout, mine);
+  if (low) atomicAdd(&nbelow, low);
+  uint32_t w = block_scan(mine, wsum);  // (its barriers also finish the bitmap)
+  for (int i = 0; i < HASH / TS; ++i) {
+    const uint32_t k = keys[t * (HASH / TS) + i];
+    if (k == EMPTY) continue;
+    const int64_t r = (int64_t)k - wlo;
+    if (!use_bm || r < 0 || r >= WB) list[w++] = k;
+  }
+  static_assert(WB / 32 == 2 * TS, "two bitmap words per thread");
+  if (use_bm) {
+    const uint32_t c0 = __popc(bm[2 * t]), c1 = __popc(bm[2 * t + 1]);
+    const uint32_t pw = block_scan(c0 + c1, wsum);
+    pre[2 * t] = pw;
+    pre[2 * t + 1] = pw + c0;
   }
   __syncthreads();
+  const uint32_t mo = nout, mb = nbelow, mw = m - mo;
   static_assert(TS == 256 && 8 * TS >= TM_TILE_MAX && SORTN >= 8 * TS, "tile_sort sizes");
-  if (m <= 2 * TS) tile_sort<2>(list, m, t);
-  else if (m <= 4 * TS) tile_sort<4>(list, m, t);
-  else tile_sort<8>(list, m, t);
-  // local position of every distinct neighbour
-  for (uint32_t i = t; i < m; i += TS) {
+  if (mo == 0) {
+    // all in the window: nothing to sort
+  } else if (mo <= TS) tile_sort<1>(list, mo, t);
+  else if (mo <= 2 * TS) tile_sort<2>(list, mo, t);
+  else if (mo <= 4 * TS) tile_sort<4>(list, mo, t);
+  else tile_sort<8>(list, mo, t);
+  // local position of every distinct neighbour: the sorted ones ...
+  for (uint32_t i = t; i < mo; i += TS) {
     const uint32_t q = list[i];
+    const uint32_t pos = i < mb ? i : i + mw;
     uint32_t h = hslot(q);
     while (keys[h] != q) h = (h + 1) & (HASH - 1);
-    vals[h] = (uint16_t)i;
-    tile_idx[g * TM_TILE_MAX + i] = q;
+    vals[h] = (uint16_t)pos;
+    tile_idx[g * TM_TILE_MAX + pos] = q;
+  }
+  // ... and the window's, by their rank in the bitmap
+  for (int i = 0; use_bm && i < HASH / TS; ++i) {
+    const uint32_t hs = t * (HASH / TS) + i;
+    const uint32_t k = keys[hs];
+    if (k == EMPTY) continue;
+    const int64_t r = (int64_t)k - wlo;
+    if (use_bm && r >= 0 && r < WB) {
+      const uint32_t wd = (uint32_t)(r >> 5), bt = (uint32_t)(r & 31);
+      const uint32_t pos = mb + pre[wd] + __popc(bm[wd] & ((1u << bt) - 1u));
+      vals[hs] = (uint16_t)pos;
+      tile_idx[g * TM_TILE_MAX + pos] = k;
+    }
   }
   if (t == 0) {
     tile_cnt[g] = m;
