@@ -119,7 +119,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   __shared__ uint8_t order[TS];
   __shared__ uint32_t wsum[TS / 64];
   __shared__ uint32_t bm[WB / 32], pre[WB / 32];
-  __shared__ uint32_t distinct, nout, nbelow;
+  __shared__ uint32_t distinct, nwin, nbelow;
   const int64_t g = blockIdx.x;
   const uint32_t t = threadIdx.x;
   const int64_t cell = g * TS + t;
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   for (int i = t; i < HASH; i += TS) keys[i] = EMPTY;
   for (int i = t; i < WB / 32; i += TS) bm[i] = 0;
   bins[t] = 0;
-  if (t == 0) distinct = nout = nbelow = 0;
+  if (t == 0) distinct = nbelow = 0;
   __syncthreads();
 
   // rank the cells by list length, longest first (counting sort; the order
@@ -143,8 +143,14 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   order[rank] = (uint8_t)t;
   rcnt[rank] = cnt;
 
-  // distinct neighbours of the tile (stop counting past TM_TILE_MAX); a
-  // lane reads 8 list slots per round so their loads overlap
+  // Distinct neighbours of the tile.  Most lie within WB / 2 cells of its
+  // middle (84% in the 2M-cell Gaussian colony): those are marked in a
+  // bitmap, whose popcount prefix later gives their ascending order and
+  // their positions; the others go into the hash set and are sorted.  The
+  // set's order is: the sorted ones below the window, the window's, the
+  // sorted ones above it.  (Stop counting past TM_TILE_MAX; a lane reads 8
+  // list slots per round so their loads overlap.)
+  const int64_t wlo = g * TS + TS / 2 - WB / 2;  // may be negative
   for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
     if (*(volatile uint32_t *)&distinct > TM_TILE_MAX) break;
     uint32_t qs[8];
@@ -154,6 +160,11 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     for (int j = 0; j < 8; ++j) {
       const uint32_t q = qs[j];
       if (q == EMPTY) continue;
+      const int64_t r = (int64_t)q - wlo;
+      if (r >= 0 && r < WB) {
+        atomicOr(&bm[r >> 5], 1u << (r & 31));
+        continue;
+      }
       uint32_t h = hslot(q);
       for (;;) {
         const uint32_t old = atomicCAS(&keys[h], EMPTY, q);
@@ -167,7 +178,16 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     }
   }
   __syncthreads();
-  const uint32_t m = distinct;
+  // the window's count and prefix
+  static_assert(WB / 32 == 2 * TS, "two bitmap words per thread");
+  const uint32_t b0 = bm[2 * t], b1 = bm[2 * t + 1];
+  const uint32_t c0 = __popc(b0), c1 = __popc(b1);
+  const uint32_t pw = block_scan(c0 + c1, wsum);
+  pre[2 * t] = pw;
+  pre[2 * t + 1] = pw + c0;
+  if (t == TS - 1) nwin = pw + c0 + c1;
+  __syncthreads();
+  const uint32_t mo = distinct, mw = nwin, m = mo + mw;
   if (m > TM_TILE_MAX) {
     if (t == 0) {
       tile_cnt[g] = TM_TILE_OVER;
@@ -175,44 +195,22 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     }
     return;
   }
-  // Order the set ascending.  Most of a tile's neighbours lie within WB / 2
-  // cells of its middle (84% in the 2M-cell Gaussian colony): those are
-  // marked in a bitmap, which holds them in order; only the others are
-  // compacted and sorted.  The order is then: the sorted ones below the
-  // window, the window's, the sorted ones above it.  A set of at most TS
-  // is sorted whole (one element per thread).
-  const bool use_bm = m > (uint32_t)TS;
-  const int64_t wlo = g * TS + TS / 2 - WB / 2;  // may be negative
+  // compact the hash set in slot order, then sort it ascending
   uint32_t mine = 0, low = 0;
   for (int i = 0; i < HASH / TS; ++i) {
     const uint32_t k = keys[t * (HASH / TS) + i];
     if (k == EMPTY) continue;
-    const int64_t r = (int64_t)k - wlo;
-    if (use_bm && r >= 0 && r < WB) {
-      atomicOr(&bm[r >> 5], 1u << (r & 31));
-    } else {
-      ++mine;
-      low += r < 0;
-    }
+    ++mine;
+    low += (int64_t)k < wlo;
   }
-  if (mine) atomicAdd(&nout, mine);
   if (low) atomicAdd(&nbelow, low);
-  uint32_t w = block_scan(mine, wsum);  // (its barriers also finish the bitmap)
+  uint32_t w = block_scan(mine, wsum);
   for (int i = 0; i < HASH / TS; ++i) {
     const uint32_t k = keys[t * (HASH / TS) + i];
-    if (k == EMPTY) continue;
-    const int64_t r = (int64_t)k - wlo;
-    if (!use_bm || r < 0 || r >= WB) list[w++] = k;
-  }
-  static_assert(WB / 32 == 2 * TS, "two bitmap words per thread");
-  if (use_bm) {
-    const uint32_t c0 = __popc(bm[2 * t]), c1 = __popc(bm[2 * t + 1]);
-    const uint32_t pw = block_scan(c0 + c1, wsum);
-    pre[2 * t] = pw;
-    pre[2 * t + 1] = pw + c0;
+    if (k != EMPTY) list[w++] = k;
   }
   __syncthreads();
-  const uint32_t mo = nout, mb = nbelow, mw = m - mo;
+  const uint32_t mb = nbelow;
   static_assert(TS == 256 && 8 * TS >= TM_TILE_MAX && SORTN >= 8 * TS, "tile_sort sizes");
   if (mo == 0) {
     // all in the window: nothing to sort
@@ -220,7 +218,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   else if (mo <= 2 * TS) tile_sort<2>(list, mo, t);
   else if (mo <= 4 * TS) tile_sort<4>(list, mo, t);
   else tile_sort<8>(list, mo, t);
-  // local position of every distinct neighbour: the sorted ones ...
+  // positions: the sorted ones ...
   for (uint32_t i = t; i < mo; i += TS) {
     const uint32_t q = list[i];
     const uint32_t pos = i < mb ? i : i + mw;
@@ -229,17 +227,17 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     vals[h] = (uint16_t)pos;
     tile_idx[g * TM_TILE_MAX + pos] = q;
   }
-  // ... and the window's, by their rank in the bitmap
-  for (int i = 0; use_bm && i < HASH / TS; ++i) {
-    const uint32_t hs = t * (HASH / TS) + i;
-    const uint32_t k = keys[hs];
-    if (k == EMPTY) continue;
-    const int64_t r = (int64_t)k - wlo;
-    if (use_bm && r >= 0 && r < WB) {
-      const uint32_t wd = (uint32_t)(r >> 5), bt = (uint32_t)(r & 31);
-      const uint32_t pos = mb + pre[wd] + __popc(bm[wd] & ((1u << bt) - 1u));
-      vals[hs] = (uint16_t)pos;
-      tile_idx[g * TM_TILE_MAX + pos] = k;
+  // ... and the window's, in order (thread t holds words 2t and 2t + 1)
+  {
+    uint32_t pos = mb + pw;
+    for (int e = 0; e < 2; ++e) {
+      uint32_t bits = e ? b1 : b0;
+      const int64_t q0 = wlo + 32 * (2 * (int64_t)t + e);
+      while (bits) {
+        const uint32_t bt = (uint32_t)__ffs(bits) - 1u;
+        bits &= bits - 1u;
+        tile_idx[g * TM_TILE_MAX + pos++] = (uint32_t)(q0 + bt);
+      }
     }
   }
   if (t == 0) {
@@ -270,9 +268,15 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
       v[j] = m;
       const uint32_t q = qs[j];
       if (q != EMPTY) {
-        uint32_t h = hslot(q);
-        while (keys[h] != q) h = (h + 1) & (HASH - 1);
-        v[j] = vals[h];
+        const int64_t rr = (int64_t)q - wlo;
+        if (rr >= 0 && rr < WB) {
+          const uint32_t wd = (uint32_t)(rr >> 5), bt = (uint32_t)(rr & 31);
+          v[j] = mb + pre[wd] + __popc(bm[wd] & ((1u << bt) - 1u));
+        } else {
+          uint32_t h = hslot(q);
+          while (keys[h] != q) h = (h + 1) & (HASH - 1);
+          v[j] = vals[h];
+        }
       }
     }
     *reinterpret_cast<uint4 *>(nbr16 + tm_nbr16_at(r, k0, cap)) =
