@@ -25,12 +25,16 @@
 namespace {
 
 constexpr int TS = TM_TILE;
-constexpr int HASH = 4096;  // open-addressing set of distinct neighbours (> 2 TM_TILE_MAX)
+// open-addressing set of the distinct neighbours outside a tile's bitmap
+// window; a key reserves its place before it is inserted, so the set never
+// holds more than TM_TILE_MAX keys and every probe meets an empty slot
+constexpr int HASH = 2048;
 constexpr int SORTN = 2048;  // power of two >= TM_TILE_MAX
 constexpr int WB = 16384;    // cells around a tile's middle whose order a bitmap gives
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 20; }  // 12 bits
+__device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 21; }  // 11 bits
+static_assert(HASH == 2048 && HASH > TM_TILE_MAX, "hash slots");
 
 // Ascending bitonic sort of list[0, m) (m <= 256 E) by the 256 threads of
 // the workgroup, each holding E consecutive elements (padding EMPTY sorts
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   __shared__ uint8_t order[TS];
   __shared__ uint32_t wsum[TS / 64];
   __shared__ uint32_t bm[WB / 32], pre[WB / 32];
-  __shared__ uint32_t distinct, nwin, nbelow;
+  __shared__ uint32_t distinct, nwin, nbelow, hfull;
   const int64_t g = blockIdx.x;
   const uint32_t t = threadIdx.x;
   const int64_t cell = g * TS + t;
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   for (int i = t; i < HASH; i += TS) keys[i] = EMPTY;
   for (int i = t; i < WB / 32; i += TS) bm[i] = 0;
   bins[t] = 0;
-  if (t == 0) distinct = nbelow = 0;
+  if (t == 0) distinct = nbelow = hfull = 0;
   __syncthreads();
 
   // rank the cells by list length, longest first (counting sort; the order
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   // list slots per round so their loads overlap.)
   const int64_t wlo = g * TS + TS / 2 - WB / 2;  // may be negative
   for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
-    if (*(volatile uint32_t *)&distinct > TM_TILE_MAX) break;
+    if (*(volatile uint32_t *)&hfull) break;
     uint32_t qs[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) qs[j] = k0 + j < cnt ? nbr[tm_slot(cell, k0 + j, cap)] : EMPTY;
@@ -165,14 +169,29 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
         atomicOr(&bm[r >> 5], 1u << (r & 31));
         continue;
       }
+      // look it up first (most occurrences are repeats); slots are never
+      // emptied, so at most TM_TILE_MAX of the HASH slots read are taken
       uint32_t h = hslot(q);
+      uint32_t k = keys[h];
+      while (k != q && k != EMPTY) {
+        h = (h + 1) & (HASH - 1);
+        k = keys[h];
+      }
+      if (k == q) continue;
+      // a new key: reserve a place (the count of reserved places bounds the
+      // set), then insert from the empty slot found
+      if (atomicAdd(&distinct, 1u) >= (uint32_t)TM_TILE_MAX) {
+        atomicSub(&distinct, 1u);
+        hfull = 1u;  // over-full (or about to be): the tile keeps the 32-bit lists
+        continue;
+      }
       for (;;) {
         const uint32_t old = atomicCAS(&keys[h], EMPTY, q);
-        if (old == EMPTY) {
-          atomicAdd(&distinct, 1u);
+        if (old == EMPTY) break;
+        if (old == q) {  // another lane inserted it meanwhile
+          atomicSub(&distinct, 1u);
           break;
         }
-        if (old == q) break;
         h = (h + 1) & (HASH - 1);
       }
     }
@@ -188,7 +207,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   if (t == TS - 1) nwin = pw + c0 + c1;
   __syncthreads();
   const uint32_t mo = distinct, mw = nwin, m = mo + mw;
-  if (m > TM_TILE_MAX) {
+  if (m > TM_TILE_MAX || hfull) {
     if (t == 0) {
       tile_cnt[g] = TM_TILE_OVER;
       atomicMax(flags + 4, m);
