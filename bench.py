@@ -77,10 +77,13 @@ def traffic_from_profiles():
         return None
 
 
-def cpu_walks(col, threads):
-    """oracle octbuild + the two per-cell walks; median of 3 after a warm-up"""
+def cpu_walks(col, threads, reps=3, deposit=False):
+    """oracle octbuild + the two per-cell walks (median of `reps` after a
+    warm-up); with deposit, also the serial CIC deposit onto the colony's grid
+    (median of 3), the reference's src/patches.c:156-234"""
     from oracle import oracle as O
 
+    prev = O.max_threads()
     if threads > 0:
         O.set_threads(threads)
     nthr = O.max_threads()
@@ -91,7 +94,7 @@ def cpu_walks(col, threads):
     tree.computepotential()  # warm-up
     tree.computegradient()
     walks = []
-    for _ in range(3):
+    for _ in range(reps):
         t0 = O.wtime()
         tree.computepotential()
         t1 = O.wtime()
@@ -99,8 +102,17 @@ def cpu_walks(col, threads):
         t2 = O.wtime()
         walks.append((t1 - t0, t2 - t1))
     walks.sort(key=lambda w: w[0] + w[1])
-    tp, tg = walks[1]
-    return nthr, t_build, tp, tg
+    tp, tg = walks[len(walks) // 2]
+    t_dep = None
+    if deposit:
+        ts = []
+        for _ in range(3):
+            t0 = O.wtime()
+            O.cic_deposit(oc, col["grid"], 0)
+            ts.append(O.wtime() - t0)
+        t_dep = sorted(ts)[1]
+    O.set_threads(prev)
+    return nthr, t_build, tp, tg, t_dep
 
 
 def gpu_pairs(tm, col):
@@ -286,6 +298,7 @@ def run_single(a, tm, gen):
     build_ms = ctx.time_stage(tm.STAGE_BUILD, tm.MIXED, warmup=1, iters=3)
     nbr_ms = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
     cic_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
+    tiles_ms = ctx.time_stage(tm.STAGE_TILES, tm.MIXED, warmup=2, iters=10)
     env = environment_solve(tm, ctx, grid, prod)
     prod.free()
     copy_gbs = copy_bandwidth(tm, ctx)
@@ -305,15 +318,41 @@ def run_single(a, tm, gen):
     loop_ms = (time.perf_counter() - t0) * 1e3 / nloop
     ctx.graph_destroy()
 
-    cpu = None
+    cpu = cpu1 = None
     if not a.no_cpu:
-        nthr, t_build, tp, tg = cpu_walks(col, a.cpu_threads)
+        nthr, t_build, tp, tg, t_dep = cpu_walks(col, a.cpu_threads, deposit=True)
         cpu = {"value": P / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
                "sample": f"full workload ({n} cells, {P} pairs): computepotential + computegradient walks, "
-                         f"median of 3 after a warm-up; serial octbuild {t_build:.2f} s not included",
-               "t_potential_s": tp, "t_gradient_s": tg, "t_octbuild_s": t_build}
+                         f"median of 3 after a warm-up; serial octbuild {t_build:.2f} s and serial CIC deposit "
+                         f"{t_dep:.3f} s timed beside it (full_step_s)",
+               "t_potential_s": tp, "t_gradient_s": tg, "t_octbuild_s": t_build, "t_cic_deposit_s": t_dep,
+               "full_step_s": t_build + tp + tg + t_dep}
+        # one core, on a bounded sample: the same generator at a tenth of the
+        # cells (same density, so the same neighbours per cell)
+        smp = gen.config(1, scale=0.1 * a.scale)
+        Ps = gpu_pairs(tm, smp)
+        _, b1, p1, g1, _ = cpu_walks(smp, 1, reps=1)
+        cpu1 = {"value": Ps / (p1 + g1), "unit": "pair-interactions/s", "cores": 1, "kind": "port",
+                "sample": f"same generator at a tenth of the cells ({smp['n']} cells, {Ps} pairs, same density): "
+                          f"computepotential + computegradient walks on one thread, one run after a warm-up",
+                "t_potential_s": p1, "t_gradient_s": g1, "t_octbuild_s": b1}
     line = line_common(a, out, n, P, maxnbr)
     mixed = out.get("mixed") or out.get("fp64")
+    sph_mixed = out["mixed"]["ms"] if "mixed" in out else None
+    sph_fp64 = out["fp64"]["ms"] if "fp64" in out else None
+    # like for like: the tile build exists only to feed the pair kernel, and
+    # the CPU's two walks both find the neighbours and sum the pairs
+    lfl = {"tiles_ms": tiles_ms, "walk_ms": nbr_ms - tiles_ms, "neighbors_ms": nbr_ms}
+    for name, t in (("mixed", sph_mixed), ("fp64", sph_fp64)):
+        if t is None:
+            continue
+        lfl[name] = {"sph_incl_tiles_ms": tiles_ms + t,
+                     "pairs_per_s_incl_tiles": P / ((tiles_ms + t) * 1e-3),
+                     "neighbors_plus_sph_ms": nbr_ms + t,
+                     "pairs_per_s_neighbors_plus_sph": P / ((nbr_ms + t) * 1e-3)}
+        if cpu:
+            lfl[name]["speedup_vs_cpu_walks"] = (cpu["t_potential_s"] + cpu["t_gradient_s"]) / ((nbr_ms + t) * 1e-3)
+    gpu_full = step_ms + cic_ms
     line.update({
         "data": "synthetic (seeded Gaussian spheroid)",
         "config": {"workload": "single colony, 2,000,000 cells, Gaussian spheroid W=146.6, r=1, h=2, mixed SPH"
@@ -321,14 +360,17 @@ def run_single(a, tm, gen):
                    "max_neighbours": maxnbr, "parallelism": "1 GPU"},
         "mixed_vs_fp64_error": err,
         "interp_fp32_vs_fp64_error_256^3": ierr,
-        "full_step_ms": {"total": step_ms + cic_ms, "build": build_ms, "neighbors": nbr_ms,
-                         "sph_mixed": out["mixed"]["ms"] if "mixed" in out else None, "cic_deposit_256^3": cic_ms},
+        "full_step_ms": {"total": gpu_full, "build": build_ms, "neighbors": nbr_ms, "tiles_in_neighbors": tiles_ms,
+                         "sph_mixed": sph_mixed, "cic_deposit_256^3": cic_ms},
+        "like_for_like": lfl,
         "mechanics_loop_ms_per_step": loop_ms,
         "environment_solve_256^3": env,
         "copy_kernel_gbs": copy_gbs,
         "roofline": roofline(B, mixed["ms"], traffic_from_profiles()),
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "speedup_vs_cpu": (mixed["pairs_per_s"] / cpu["value"]) if cpu else None,
+        "speedup_full_step_vs_cpu": (cpu["full_step_s"] / (gpu_full * 1e-3)) if cpu else None,
         "autotune": tune,
     })
     line["roofline"]["frac_of_copy"] = line["roofline"]["achieved"] / copy_gbs
@@ -389,7 +431,7 @@ def run_batched(a, tm, gen):
         Pk = int(off[int(off_c[k])])
         nthr = 0
         for c in cols[:k]:
-            nthr, t_build, p_, g_ = cpu_walks(c, a.cpu_threads)
+            nthr, t_build, p_, g_, _ = cpu_walks(c, a.cpu_threads)
             tp, tg, tb = tp + p_, tg + g_, tb + t_build
         cpu = {"value": Pk / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
                "sample": f"first {k} of {len(cols)} colonies ({int(off_c[k])} cells, {Pk} pairs), "
@@ -465,7 +507,7 @@ def run_large(a, tm, gen):
     if not a.no_cpu:
         smp = gen.config(4, scale=0.04 * a.scale)
         Ps = gpu_pairs(tm, smp)
-        nthr, t_build, tp, tg = cpu_walks(smp, a.cpu_threads)
+        nthr, t_build, tp, tg, _ = cpu_walks(smp, a.cpu_threads)
         cpu = {"value": Ps / (tp + tg), "unit": "pair-interactions/s", "cores": nthr, "kind": "port",
                "sample": f"same generator at 4% scale ({smp['n']} cells, {Ps} pairs, same density and type mix): "
                          f"computepotential + computegradient walks, median of 3",

@@ -195,6 +195,60 @@ typedef struct solverdata_t {
 /* src/global.h:408-412 */
 typedef struct octheap_t { int size, count; int *data; } octheap_t;
 
+/* Layout guards (LP64: x86-64, AArch64).  A program built against the
+ * reference's src/global.h passes these structures by pointer and by value,
+ * so their sizes and the offsets of the members this library reads or writes
+ * must be the reference's: any drift fails the build instead of corrupting
+ * cells at run time. */
+#if defined(__LP64__) && (defined(__x86_64__) || defined(__aarch64__))
+#include <stddef.h>
+#ifdef __cplusplus
+#define TM_LAYOUT_ASSERT(c, m) static_assert(c, m)
+#else
+#define TM_LAYOUT_ASSERT(c, m) _Static_assert(c, m)
+#endif
+/* celldata_t, src/global.h:100-120 */
+TM_LAYOUT_ASSERT(sizeof(celldata_t) == 112, "celldata_t is 112 bytes");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, phase) == 4, "celldata_t.phase at 4");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, young) == 36, "celldata_t.young at 36");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, x) == 48, "celldata_t.x at 48");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, size) == 72, "celldata_t.size at 72");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, v) == 80, "celldata_t.v at 80");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, density) == 88, "celldata_t.density at 88");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, mindist) == 96, "celldata_t.mindist at 96");
+TM_LAYOUT_ASSERT(offsetof(celldata_t, ctype) == 104, "celldata_t.ctype at 104");
+/* octnode_t, src/global.h:153-164 */
+TM_LAYOUT_ASSERT(sizeof(octnode_t) == 112, "octnode_t is 112 bytes");
+TM_LAYOUT_ASSERT(offsetof(octnode_t, level) == 24, "octnode_t.level at 24");
+TM_LAYOUT_ASSERT(offsetof(octnode_t, father) == 32, "octnode_t.father at 32");
+TM_LAYOUT_ASSERT(offsetof(octnode_t, child) == 40, "octnode_t.child at 40");
+TM_LAYOUT_ASSERT(offsetof(octnode_t, data) == 104, "octnode_t.data at 104");
+/* cellsinfo_t, src/global.h:166-179 */
+TM_LAYOUT_ASSERT(sizeof(cellsinfo_t) == 192, "cellsinfo_t is 192 bytes");
+TM_LAYOUT_ASSERT(offsetof(cellsinfo_t, cells) == 136, "cellsinfo_t.cells at 136");
+TM_LAYOUT_ASSERT(offsetof(cellsinfo_t, forces) == 144, "cellsinfo_t.forces at 144");
+TM_LAYOUT_ASSERT(offsetof(cellsinfo_t, octree) == 152, "cellsinfo_t.octree at 152");
+TM_LAYOUT_ASSERT(offsetof(cellsinfo_t, octsize) == 160, "cellsinfo_t.octsize at 160");
+TM_LAYOUT_ASSERT(offsetof(cellsinfo_t, dimension) == 176, "cellsinfo_t.dimension at 176");
+/* grid_t, src/global.h:182-190 */
+TM_LAYOUT_ASSERT(sizeof(grid_t) == 136, "grid_t is 136 bytes");
+TM_LAYOUT_ASSERT(offsetof(grid_t, localsize) == 72, "grid_t.localsize at 72");
+TM_LAYOUT_ASSERT(offsetof(grid_t, resolution) == 96, "grid_t.resolution at 96");
+TM_LAYOUT_ASSERT(offsetof(grid_t, loweridx) == 104, "grid_t.loweridx at 104");
+/* celltype_t, src/global.h:315-335 */
+TM_LAYOUT_ASSERT(sizeof(celltype_t) == 368, "celltype_t is 368 bytes");
+TM_LAYOUT_ASSERT(offsetof(celltype_t, size) == 152, "celltype_t.size at 152");
+TM_LAYOUT_ASSERT(offsetof(celltype_t, h) == 160, "celltype_t.h at 160");
+TM_LAYOUT_ASSERT(offsetof(celltype_t, h4) == 184, "celltype_t.h4 at 184");
+TM_LAYOUT_ASSERT(offsetof(celltype_t, randommove) == 360, "celltype_t.randommove at 360");
+/* environment_t, src/global.h:354-364; cellenvdata_t, src/global.h:122-127 */
+TM_LAYOUT_ASSERT(sizeof(environment_t) == 192, "environment_t is 192 bytes");
+TM_LAYOUT_ASSERT(offsetof(environment_t, data) == 168, "environment_t.data at 168");
+TM_LAYOUT_ASSERT(offsetof(environment_t, production) == 184, "environment_t.production at 184");
+TM_LAYOUT_ASSERT(sizeof(cellenvdata_t) == 32, "cellenvdata_t is 32 bytes");
+#undef TM_LAYOUT_ASSERT
+#endif
+
 /* ---- src/octree.h ---- */
 extern double3dv_t affShift;
 extern double affScale;

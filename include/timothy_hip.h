@@ -67,7 +67,8 @@ typedef enum tm_stage {
   TM_STAGE_NEIGHBORS = 1,
   TM_STAGE_SPH = 2,
   TM_STAGE_CIC = 3,
-  TM_STAGE_STEP = 4
+  TM_STAGE_STEP = 4,
+  TM_STAGE_TILES = 5 /* the LDS tiles alone (part of tm_neighbors) */
 } tm_stage;
 
 /* mechanics fields of celltype_t (src/global.h:315-335), as given:
@@ -115,9 +116,13 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * cells' leaves (0, 512, 1024, 2048; default 1024), "walk_f32" (1 = the walk
  * screens candidates with FP32 positions relative to the colony's corner and
  * a proved error bound, reading FP64 positions only where that bound cannot
- * decide; results are identical; default 1); read-only "nbr_alloc"
- * (slots per cell of the current lists) and
- * "tile_lds" (records staged per tile, 0 without tiles) */
+ * decide; results are identical; default 1), "tile_window" (1 = the tile
+ * build orders the distinct neighbours within 8192 cells of a tile's middle
+ * by bitmap and hashes only the rest; 0 = hashes all of them; results are
+ * identical; default 1); read-only "nbr_alloc" (slots per cell of the
+ * current lists), "tile_lds" (records staged per tile, 0 without tiles) and
+ * "tiles_over" (tiles with more than 1536 distinct neighbours, which take
+ * the 32-bit lists, at the last tm_neighbors) */
 tm_status tm_set_param(tm_ctx *ctx, const char *key, int64_t value);
 tm_status tm_get_param(tm_ctx *ctx, const char *key, int64_t *value);
 

@@ -20,6 +20,8 @@ def _declared(header, pattern):
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     txt = re.sub(r"//[^\n]*", "", txt)
     txt = re.sub(r"typedef[^;]*;", "", txt, flags=re.S)
+    txt = re.sub(r"^[ \t]*#[^\n]*(\\\n[^\n]*)*", "", txt, flags=re.M)  # preprocessor lines
+    txt = re.sub(r"TM_LAYOUT_ASSERT\([^;]*\);", "", txt)  # compile-time layout checks
     return sorted(set(re.findall(pattern, txt)))
 
 

@@ -186,3 +186,22 @@ def test_updatepositions_after_step(oracle_mod, tmp_path):
     # writevtk after the update (settings.step 1): moved positions, the step's forces
     vtk = open(tmp_path / "vtk" / "step00000001.vtk", "rb").read()
     assert vtk == O.writevtk(str(tmp_path / "ref.vtk"), ex, ey, ez, col["ctype"], F)
+
+
+def test_step_compute_with_no_cells(oracle_mod, tmp_path):
+    """a rank whose cells have all left (localcount.n == 0): octbuild returns
+    without a tree (src/octree.c:152-153), the CIC deposit adds nothing, so
+    every field's production is zero (src/patches.c:172 loops over no
+    cells, src/patches.c:327-328 zeroes production first), the interpolation
+    has no cells to fill, and the field gradients are still computed"""
+    O = oracle_mod
+    G = 16
+    grid = dict(lower=np.array([-10.0, -12.0, -8.0]), resolution=np.float32(1.5), loweridx=np.zeros(3, np.int64),
+                upperidx=np.full(3, G - 1, np.int64), localsize=np.full(3, G, np.int64), G=G)
+    col = dict(n=0, x=np.zeros(0), y=np.zeros(0), z=np.zeros(0), size=np.zeros(0), young=np.zeros(0, np.float32),
+               ctype=np.zeros(0, np.int32), phase=np.zeros(0, np.int32), types=gen.CELLTYPES, grid=grid)
+    got, fields, grads = _run(tmp_path, col, "fp64")
+    assert got["octsize"] == 0 and got["cells"].shape == (0, 6) and got["ced"].shape == (2, 0, 4)
+    assert np.all(got["prod"] == 0)
+    for k in range(len(fields)):
+        assert np.array_equal(got["grad"][k], O.fieldgradient(grid["localsize"], grid["resolution"], fields[k]))

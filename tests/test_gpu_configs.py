@@ -1,0 +1,300 @@
+"""GPU parity of the configurations' defining legs against the CPU oracle.
+
+- The 50M-cell workload's FP32 field leg: the field gradient and the CIC
+  interpolation of a seeded smooth FP32 field on a grid spanning a colony of
+  that workload's generator, against the oracle on the same field upcast to
+  FP64 (src/fieldgradient.c:149-224, src/patches.c:537-586).
+- 2M cells of that generator through the LDS-tile path with and without the
+  occupancy cap, against the oracle's walks (src/potential.c:87-138,209-259).
+- Batched colonies in mixed precision, each colony against its own oracle run.
+- The tile build's hash set filled close to its limit.
+- Cells whose only neighbour sits just inside the cutoff, in mixed precision.
+"""
+import numpy as np
+import pytest
+
+from timothy_amd import generators as gen
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx(tm):
+    c = tm.Context(0)
+    yield c
+    c.close()
+
+
+def oracle_results(tree):
+    dens, v, md = tree.computepotential()
+    F = tree.computegradient()
+    return dict(density=dens, v=v, mindist=md, force=F)
+
+
+def pair_abs_force(col, off, nbr):
+    """sum_q |f_pq| per cell (src/inline.h:59-83 without the sign), the
+    normaliser of the mixed form's force error"""
+    n = col["n"]
+    types = col["types"]
+    P = np.stack([col["x"], col["y"], col["z"]], 1)
+    out = np.zeros(n)
+    ct = np.asarray(col["ctype"])
+    h = np.array([t["h"] for t in types])
+    h4 = np.array([t["h4"] for t in types])
+    step = 1 << 20
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        cnt = np.diff(off[s:e + 1])
+        rows = np.repeat(np.arange(s, e), cnt)
+        q = nbr[off[s]:off[e]]
+        d = np.sqrt(((P[rows] - P[q]) ** 2).sum(1))
+        u = d / h[ct[rows]]
+        w = np.where(u <= 0.5, -2 * u + 3 * u * u, -(1 - u) ** 2)
+        out[s:e] = np.bincount(rows - s, np.abs(w) * 48 / (np.pi * h4[ct[rows]]), minlength=e - s)
+    return out
+
+
+def mixed_errors(col, off, nbr, got, ref):
+    sabs = pair_abs_force(col, off, nbr)
+    has = sabs > 0
+    ferr = np.abs(got["force"] - ref["force"]).max(axis=1)[has] / sabs[has]
+    pos = ref["density"] > 0
+    derr = np.abs(got["density"] - ref["density"])[pos] / ref["density"][pos]
+    return (float(ferr.max()) if ferr.size else 0.0), (float(derr.max()) if derr.size else 0.0)
+
+
+def test_c4_fp32_field_gradient_and_interp(tm, oracle_mod, ctx):
+    """FP32 field on a 128^3 grid spanning 200,000 cells of the 50M-cell
+    workload's generator (5% radius 1.5).  Gradient: the GPU forms
+    RN32(RN32(f[i+1] - f[i-1]) / res), so against the oracle's FP64 gradient
+    of the upcast field each component is within 2^-23 of its magnitude (two
+    FP32 roundings), and it equals numpy's FP32 evaluation of the same
+    expression bit for bit.  Interpolation: FP64 sums of the FP32 nodes in
+    the reference's corner order, bit-exact with the oracle fed the same
+    (upcast) field and gradient."""
+    O = oracle_mod
+    col = gen.config(4, scale=0.004)
+    grid = gen.bbox_grid(col, 128)
+    G = grid["G"]
+    say(f"C4 FP32 field leg: n={col['n']} grid {G}^3")
+    ctx.upload_colony(col)
+    ctx.build()
+    f32 = gen.smooth_field(grid, 5, np.float32)
+    f_d = tm.DeviceArray.from_host(ctx, f32)
+    g_d = tm.DeviceArray(ctx, 3 * G ** 3, np.float32)
+    ctx.field_gradient(grid, f_d, g_d)
+    g32 = g_d.get()
+    g64 = O.fieldgradient(grid["localsize"], grid["resolution"], f32.astype(np.float64))
+    assert np.all(np.abs(g32.astype(np.float64) - g64) <= 2.0 ** -23 * np.abs(g64)), "FP32 gradient beyond 2 roundings"
+    # numpy's FP32 stencil (correctly rounded subtraction and division)
+    F = f32.reshape(G, G, G)
+    r = np.float32(grid["resolution"])
+    exp = np.empty((G, G, G, 3), np.float32)
+    for a in range(3):
+        Fm = np.moveaxis(F, a, 0)
+        E = np.moveaxis(exp[..., a], a, 0)
+        E[1:-1] = (Fm[2:] - Fm[:-2]) / r
+        E[0] = (Fm[1] - Fm[0]) / r
+        E[-1] = (Fm[-1] - Fm[-2]) / r
+    assert np.array_equal(g32, exp.reshape(-1))
+    out = tm.DeviceArray(ctx, 4 * col["n"], np.float64)
+    ctx.cic_interp(grid, f_d, g_d, out)
+    got = out.get().reshape(-1, 4)
+    oc = O.Colony.from_dict(col)
+    ref = O.cic_interp(oc, grid, f32.astype(np.float64), g32.astype(np.float64))
+    assert np.array_equal(got, ref)
+    # and against the oracle's own FP64 gradient: within the gradient's bound
+    ref64 = O.cic_interp(oc, grid, f32.astype(np.float64), g64)
+    gmax = np.abs(g64).max()
+    assert np.abs(got[:, 1:] - ref64[:, 1:]).max() <= 2.0 ** -23 * gmax
+    assert np.array_equal(got[:, 0], ref64[:, 0])
+    for d in (f_d, g_d, out):
+        d.free()
+
+
+def test_c4_fp32_gradient_512(tm, ctx):
+    """the FP32 gradient at the 50M-cell workload's full 512^3 size equals
+    numpy's FP32 evaluation of the reference's stencil bit for bit"""
+    col = gen.config(4, scale=0.0002)
+    grid = gen.bbox_grid(col, 512)
+    G = grid["G"]
+    f32 = gen.smooth_field(grid, 5, np.float32)
+    f_d = tm.DeviceArray.from_host(ctx, f32)
+    g_d = tm.DeviceArray(ctx, 3 * G ** 3, np.float32)
+    say(f"FP32 gradient {G}^3")
+    ctx.field_gradient(grid, f_d, g_d)
+    g32 = g_d.get().reshape(G, G, G, 3)
+    F = f32.reshape(G, G, G)
+    r = np.float32(grid["resolution"])
+    for a in range(3):
+        Fm = np.moveaxis(F, a, 0)
+        E = np.moveaxis(g32[..., a], a, 0)
+        assert np.array_equal(E[1:-1], (Fm[2:] - Fm[:-2]) / r), a
+        assert np.array_equal(E[0], (Fm[1] - Fm[0]) / r), a
+        assert np.array_equal(E[-1], (Fm[-1] - Fm[-2]) / r), a
+    f_d.free()
+    g_d.free()
+
+
+def test_c4_tile_cap_against_oracle(tm, oracle_mod, ctx):
+    """2,000,000 cells of the 50M-cell workload's generator (5% radius 1.5):
+    some tiles need more records than the occupancy cap stages, and with the
+    cap on they take the global-record path.  FP64 results are bit-exact with
+    the oracle with the cap on and off, mixed within 1e-5."""
+    O = oracle_mod
+    col = gen.config(4, scale=0.04)
+    say(f"C4 generator n={col['n']}: oracle walks")
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    ref = oracle_results(tree)
+    del tree
+    ctx.upload_colony(col)
+    try:
+        for cap in (1, 0):
+            ctx.set_param("tile_cap4", cap)
+            ctx.step(tm.FP64)
+            lds = ctx.get_param("tile_lds")
+            say(f"tile_cap4 {cap}: staged records per tile {lds}, over-full tiles {ctx.get_param('tiles_over')}")
+            if cap:
+                assert lds == 1280  # the capped path is taken
+            else:
+                assert lds > 1280
+            r = ctx.results()
+            for k in ("density", "v", "mindist", "force"):
+                assert np.array_equal(r[k], ref[k]), (cap, k)
+            ctx.sph(tm.MIXED)
+            m = ctx.results()
+            off, nbr = ctx.neighbor_lists()
+            fe, de = mixed_errors(col, off, nbr, m, ref)
+            say(f"  mixed: force {fe:.2e} density {de:.2e}")
+            assert fe <= 1e-5 and de <= 1e-5
+            assert np.array_equal(m["mindist"], ref["mindist"])
+    finally:
+        ctx.set_param("tile_cap4", 1)
+
+
+def test_batched_colonies_mixed_per_colony(tm, oracle_mod, ctx):
+    """24 colonies of the batched workload's generator in one set of
+    launches, mixed precision: every colony within 1e-5 of its own FP64
+    oracle run (force normalised per cell by sum_q |f_pq|, density
+    relative), mindist exact; and FP64 bit-exact per colony"""
+    O = oracle_mod
+    cols = gen.colonies_config3(ncol=24, scale=0.1)
+    say(f"batched mixed n={[c['n'] for c in cols]}")
+    off = ctx.upload_colonies(cols)
+    ctx.step(tm.FP64)
+    r64 = ctx.results()
+    ctx.sph(tm.MIXED)
+    rm = ctx.results()
+    worst = (0.0, 0.0)
+    for i, c in enumerate(cols):
+        sl = slice(off[i], off[i + 1])
+        tree = O.Octree(O.Colony.from_dict(c), c["types"])
+        ref = oracle_results(tree)
+        for k in ("density", "v", "mindist", "force"):
+            assert np.array_equal(r64[k][sl], ref[k]), (i, k)
+        coff, cand, inh = tree.candidates()
+        cum = np.concatenate([[0], np.cumsum(inh, dtype=np.int64)])
+        loff = cum[coff]
+        lst = cand[inh == 1]
+        got = {k: rm[k][sl] for k in rm}
+        fe, de = mixed_errors(c, loff, lst, got, ref)
+        worst = (max(worst[0], fe), max(worst[1], de))
+        assert fe <= 1e-5 and de <= 1e-5, (i, fe, de)
+        assert np.array_equal(got["mindist"], ref["mindist"]), i
+    say(f"  worst colony: force {worst[0]:.2e} density {worst[1]:.2e}")
+
+
+def test_tile_hash_set_near_its_limit(tm, oracle_mod, ctx):
+    """40,000 cells with cutoff 5.4: tiles hold up to ~1,650 distinct
+    neighbours.  With tile_window 0 every distinct neighbour goes through the
+    tile build's hash set, so tiles with 1,281..1,536 keys stress its
+    reservation bound: exactly the tiles with more than 1,536 distinct
+    neighbours are over-full (counted on the CPU from the lists), FP64 is
+    bit-exact with the oracle, and the mixed sums are identical to those of
+    the default (bitmap) tile build and within 1e-5 of the oracle."""
+    O = oracle_mod
+    col = gen.ball_colony(40000, 5)
+    h = 5.4
+    col["types"] = [dict(size=1.0, h=h, h2=h * h, h3=h ** 3, h4=h ** 4)]
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    ref = oracle_results(tree)
+    ctx.upload_colony(col)
+    out = {}
+    try:
+        for win in (0, 1):
+            ctx.set_param("tile_window", win)
+            ctx.step(tm.FP64)
+            over = ctx.get_param("tiles_over")
+            r = ctx.results()
+            for k in ("density", "v", "mindist", "force"):
+                assert np.array_equal(r[k], ref[k]), (win, k)
+            ctx.sph(tm.MIXED)
+            out[win] = (over, ctx.results())
+            say(f"tile_window {win}: over-full tiles {over}, staged records {ctx.get_param('tile_lds')}")
+    finally:
+        ctx.set_param("tile_window", 1)
+    # distinct neighbours per tile of 256 Morton-consecutive cells
+    perm, _ = ctx.order()
+    n = col["n"]
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    off, nbr = ctx.neighbor_lists()
+    rows = np.repeat(np.arange(n), np.diff(off))
+    u = np.unique((inv[rows] // 256) * n + inv[nbr])
+    m = np.bincount(u // n, minlength=(n + 255) // 256)
+    say(f"  tiles {len(m)}: {int(((m > 1280) & (m <= 1536)).sum())} with 1281..1536 distinct, "
+        f"{int((m > 1536).sum())} above")
+    assert ((m > 1280) & (m <= 1536)).sum() >= 20
+    assert out[0][0] == out[1][0] == int((m > 1536).sum())
+    for k in out[0][1]:
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+    fe, de = mixed_errors(col, off, nbr, out[0][1], ref)
+    assert fe <= 1e-5 and de <= 1e-5
+
+
+def test_mixed_cells_with_one_neighbour_near_cutoff(tm, oracle_mod, ctx):
+    """isolated pairs of cells at separation h (1 - eps) for eps from 1e-1 to
+    1e-9 (each cell's only neighbour sits just inside the cutoff, where the
+    kernel terms vanish as eps^2 and eps^3), plus triplets with one close and
+    one near-cutoff neighbour: mixed force within 1e-5 of each cell's
+    sum_q |f_pq|, density within 1e-5, against the oracle"""
+    O = oracle_mod
+    rng = np.random.default_rng(17)
+    pts = []
+    k = 0
+    for eps in (1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8, 1e-9):
+        for rep in range(40):
+            base = np.array([(k % 20) * 30.0, ((k // 20) % 20) * 30.0, (k // 400) * 30.0]) + rng.uniform(-3, 3, 3)
+            k += 1
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            d = 2.0 * (1.0 - eps)
+            pts.append(base)
+            pts.append(base + d * v)
+            if rep % 4 == 0:  # a third cell close to the first
+                w = rng.normal(size=3)
+                w /= np.linalg.norm(w)
+                pts.append(base + rng.uniform(0.3, 1.5) * w)
+    X = np.array(pts)
+    n = len(X)
+    col = dict(n=n, x=X[:, 0].copy(), y=X[:, 1].copy(), z=X[:, 2].copy(), size=np.ones(n),
+               young=np.full(n, 2100.0, np.float32), ctype=np.zeros(n, np.int32), phase=np.ones(n, np.int32),
+               types=[dict(gen.TYPE0)])
+    ctx.upload_colony(col)
+    ctx.step(tm.FP64)
+    r64 = ctx.results()
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    ref = oracle_results(tree)
+    for k in ("density", "v", "mindist", "force"):
+        assert np.array_equal(r64[k], ref[k]), k
+    ctx.sph(tm.MIXED)
+    m = ctx.results()
+    off, nbr = ctx.neighbor_lists()
+    assert np.all(np.diff(off) >= 1)
+    fe, de = mixed_errors(col, off, nbr, m, ref)
+    say(f"near-cutoff pairs n={n}: mixed force {fe:.2e} density {de:.2e}")
+    assert fe <= 1e-5 and de <= 1e-5

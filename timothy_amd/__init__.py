@@ -20,7 +20,7 @@ TM_OK = 0
 STATUS = {0: "TM_OK", 1: "TM_ERR_INVALID", 2: "TM_ERR_HIP", 3: "TM_ERR_NOMEM", 4: "TM_ERR_DEGENERATE",
           5: "TM_ERR_STATE", 6: "TM_ERR_UNSUPPORTED"}
 FP64, MIXED = 0, 1
-STAGE_BUILD, STAGE_NEIGHBORS, STAGE_SPH, STAGE_CIC, STAGE_STEP = 0, 1, 2, 3, 4
+STAGE_BUILD, STAGE_NEIGHBORS, STAGE_SPH, STAGE_CIC, STAGE_STEP, STAGE_TILES = 0, 1, 2, 3, 4, 5
 
 
 class TMError(RuntimeError):

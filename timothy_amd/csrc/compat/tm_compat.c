@@ -199,7 +199,15 @@ static void ensure_host(systeminfo_t si, int64_t n) {
 
 void octbuild(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype) {
   const int64_t n = (int64_t)ci->localcount.n;
-  if (n == 0) return; /* src/octree.c:152-153 */
+  if (n == 0) {
+    /* src/octree.c:152-153: no tree; the step's CIC calls still run (they
+     * deposit nothing and interpolate onto no cells) */
+    S.built_for = ci;
+    S.n = 0;
+    S.have_forces = 0;
+    S.deposited = 0;
+    return;
+  }
   tm_ctx *c = ctx_for(si);
   ensure_host(si, n);
   int maxt = 0;
@@ -753,9 +761,13 @@ void patches_cells2envinit(systeminfo_t si, settings_t st, patches_t *pt, cellsi
   if (st.numberoffields == 0) return;
   ctx_for(si);
   require_build(si, ci);
-  ensure_grid(si, grid_nodes(grid));
+  const int64_t nodes = grid_nodes(grid);
+  ensure_grid(si, nodes);
   tm_grid g = grid_of(grid, si.rank);
-  check(si, tm_cic_deposit(S.ctx, &g, 0, S.d_prod), "CIC deposit");
+  if (S.n > 0)
+    check(si, tm_cic_deposit(S.ctx, &g, 0, S.d_prod), "CIC deposit");
+  else /* no cells: nothing deposited (src/patches.c:172 loops over no cells) */
+    check(si, tm_memset(S.ctx, S.d_prod, 0, sizeof(double) * (size_t)nodes), "clear production");
   pt->receiver = calloc(1, sizeof(int));
   pt->sender = calloc(1, sizeof(int));
   if (!pt->receiver || !pt->sender) FAIL(si, "cannot allocate patches");
@@ -837,7 +849,10 @@ void patches_env2cellswait(systeminfo_t si, settings_t st, patches_t *pt, cellsi
     FAIL(si, "cannot allocate cellenvdata");
   tm_grid g = grid_of(grid, si.rank);
   for (int f = 0; f < st.numberoffields; f++) {
-    if (!((*ced)[f] = (cellenvdata_t *)calloc(n, sizeof(cellenvdata_t)))) FAIL(si, "cannot allocate cellenvdata[f]");
+    /* (at least one element, so a cell-less rank still gets a pointer to free) */
+    if (!((*ced)[f] = (cellenvdata_t *)calloc(n > 0 ? n : 1, sizeof(cellenvdata_t))))
+      FAIL(si, "cannot allocate cellenvdata[f]");
+    if (n == 0) continue;
     check(si, tm_cic_interp(S.ctx, &g, S.d_envf[f], S.d_envg[f], S.d_interp), "CIC interpolation");
     check(si, tm_memcpy_dtoh(S.ctx, (*ced)[f], S.d_interp, sizeof(cellenvdata_t) * n), "cellenvdata");
   }

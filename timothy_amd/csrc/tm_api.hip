@@ -236,6 +236,8 @@ tm_status run_stage(tm_ctx *c, tm_stage s, tm_precision p) {
       TM_TRY(tm_run_build(c));
       TM_TRY(tm_run_neighbors(c, false));
       return tm_run_sph(c, p);
+    case TM_STAGE_TILES:
+      return tm_run_tiles(c, false);
   }
   return tm_fail(c, TM_ERR_INVALID, "unknown stage");
 }
@@ -419,6 +421,8 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->tile_cap4 = v ? 1 : 0;
   } else if (!strcmp(key, "walk_f32")) {
     c->walk_f32 = v ? 1 : 0;
+  } else if (!strcmp(key, "tile_window")) {
+    c->tile_window = v ? 1 : 0;
   } else if (!strcmp(key, "walk_window")) {
     if (v != 0 && v != 512 && v != 1024 && v != 2048)
       return tm_fail(c, TM_ERR_INVALID, "walk_window in {0,512,1024,2048}");
@@ -442,7 +446,9 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "tile_cap4")) *v = c->tile_cap4;
   else if (!strcmp(key, "walk_window")) *v = c->walk_window;
   else if (!strcmp(key, "walk_f32")) *v = c->walk_f32;
+  else if (!strcmp(key, "tile_window")) *v = c->tile_window;
   else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
+  else if (!strcmp(key, "tiles_over")) *v = c->tiles_over;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
   else return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
   return TM_OK;
@@ -685,7 +691,7 @@ tm_status tm_time_stage(tm_ctx *c, tm_stage s, tm_precision p, int32_t warmup, i
   // the neighbour lists must have been sized by one checked tm_neighbors call
   if (s == TM_STAGE_NEIGHBORS && (c->stage < 2 || c->nbr_alloc == 0))
     return tm_fail(c, TM_ERR_STATE, "run tm_neighbors once before timing");
-  if ((s == TM_STAGE_SPH || s == TM_STAGE_STEP) && c->stage < 3)
+  if ((s == TM_STAGE_SPH || s == TM_STAGE_STEP || s == TM_STAGE_TILES) && c->stage < 3)
     return tm_fail(c, TM_ERR_STATE, "run tm_neighbors once before timing");
   for (int i = 0; i < warmup; ++i) TM_TRY(run_stage(c, s, p));
   TM_HIP(c, hipEventRecord(c->ev0, c->stream));

@@ -144,6 +144,8 @@ struct tm_ctx {
   // LDS tiles built from the lists (see TM_TILE)
   int sph_tiles = 1;
   int tile_cap4 = 1;  // cap the staged records at TM_TILE_LDS4 when few tiles are larger
+  int tile_window = 1;  // k_tiles orders the neighbours near a tile's middle by bitmap (0: all by hash set)
+  int64_t tiles_over = 0;  // over-full tiles at the last checked tile build
   uint32_t *d_tile_idx = nullptr;   // per tile: distinct neighbours, ascending (TM_TILE_MAX slots)
   uint32_t *d_tile_cnt = nullptr;   // per tile: their number, or TM_TILE_OVER
   uint8_t *d_tile_cell = nullptr;   // per tile: cell offset of each rank (longest list first)
@@ -165,7 +167,8 @@ struct tm_ctx {
   // [4] largest tile, [5] tiles above TM_TILE_LDS4 records, [6..7] cells newly
   // marked by the last eager move (u64),
   // [8..9] the same summed over captured-loop replays, [10..11] the captured
-  // loop's next Philox step, [12..13] error bits of the moves
+  // loop's next Philox step, [12..13] error bits of the moves, [14] over-full
+  // tiles
   unsigned int *d_flags = nullptr;
   unsigned int *h_flags = nullptr;  // pinned
 

@@ -25,16 +25,24 @@
 namespace {
 
 constexpr int TS = TM_TILE;
-// open-addressing set of the distinct neighbours outside a tile's bitmap
-// window; a key reserves its place before it is inserted, so the set never
-// holds more than TM_TILE_MAX keys and every probe meets an empty slot
+// Open-addressing set of the distinct neighbours outside a tile's bitmap
+// window.  A new key reserves a place before it is inserted, and a
+// reservation is granted only while fewer than RESERVE places are held.
+// Each lane holds at most one reservation that is not yet a key of the set
+// (two lanes may race to insert the same key; the loser gives its place
+// back), so the held places never exceed the keys inserted plus TS - 1.
+// Hence: the set never holds more than RESERVE keys, so at least
+// HASH - RESERVE slots stay empty and every probe ends; and a refused
+// reservation means at least RESERVE - (TS - 1) = TM_TILE_MAX + 1 keys are
+// in the set, i.e. the tile is truly over-full.
 constexpr int HASH = 2048;
+constexpr uint32_t RESERVE = TM_TILE_MAX + TS;
 constexpr int SORTN = 2048;  // power of two >= TM_TILE_MAX
 constexpr int WB = 16384;    // cells around a tile's middle whose order a bitmap gives
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 21; }  // 11 bits
-static_assert(HASH == 2048 && HASH > TM_TILE_MAX, "hash slots");
+static_assert(HASH == 2048 && HASH > (int)RESERVE, "hash slots");
 
 // Ascending bitonic sort of list[0, m) (m <= 256 E) by the 256 threads of
 // the workgroup, each holding E consecutive elements (padding EMPTY sorts
@@ -111,7 +119,8 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *wsum) {
   return base + inc - v;
 }
 
-__global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint32_t *__restrict__ nbr,
+// win: WB, or 0 to send every key through the hash set (same outputs)
+__global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t win, const uint32_t *__restrict__ nbr,
                                               const uint32_t *__restrict__ count, uint32_t *__restrict__ tile_idx,
                                               uint32_t *__restrict__ tile_cnt, uint8_t *__restrict__ tile_cell,
                                               uint16_t *__restrict__ nbr16, unsigned int *__restrict__ flags) {
@@ -154,7 +163,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   // set's order is: the sorted ones below the window, the window's, the
   // sorted ones above it.  (Stop counting past TM_TILE_MAX; a lane reads 8
   // list slots per round so their loads overlap.)
-  const int64_t wlo = g * TS + TS / 2 - WB / 2;  // may be negative
+  const int64_t wlo = g * TS + TS / 2 - (int64_t)(win / 2);  // may be negative
   for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
     if (*(volatile uint32_t *)&hfull) break;
     uint32_t qs[8];
@@ -165,7 +174,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
       const uint32_t q = qs[j];
       if (q == EMPTY) continue;
       const int64_t r = (int64_t)q - wlo;
-      if (r >= 0 && r < WB) {
+      if (r >= 0 && r < (int64_t)win) {
         atomicOr(&bm[r >> 5], 1u << (r & 31));
         continue;
       }
@@ -180,9 +189,9 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
       if (k == q) continue;
       // a new key: reserve a place (the count of reserved places bounds the
       // set), then insert from the empty slot found
-      if (atomicAdd(&distinct, 1u) >= (uint32_t)TM_TILE_MAX) {
+      if (atomicAdd(&distinct, 1u) >= RESERVE) {
         atomicSub(&distinct, 1u);
-        hfull = 1u;  // over-full (or about to be): the tile keeps the 32-bit lists
+        hfull = 1u;  // more than TM_TILE_MAX keys: the tile keeps the 32-bit lists
         continue;
       }
       for (;;) {
@@ -211,6 +220,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
     if (t == 0) {
       tile_cnt[g] = TM_TILE_OVER;
       atomicMax(flags + 4, m);
+      atomicAdd(flags + 14, 1u);
     }
     return;
   }
@@ -288,7 +298,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
       const uint32_t q = qs[j];
       if (q != EMPTY) {
         const int64_t rr = (int64_t)q - wlo;
-        if (rr >= 0 && rr < WB) {
+        if (rr >= 0 && rr < (int64_t)win) {
           const uint32_t wd = (uint32_t)(rr >> 5), bt = (uint32_t)(rr & 31);
           v[j] = mb + pre[wd] + __popc(bm[wd] & ((1u << bt) - 1u));
         } else {
@@ -303,9 +313,11 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, const uint
   }
 }
 
-__global__ void k_zero_two(unsigned int *p) {
-  p[0] = 0;
-  p[1] = 0;
+// clears the tile counters: flags[4], flags[5] and flags[14]
+__global__ void k_zero_tile_flags(unsigned int *flags) {
+  flags[4] = 0;
+  flags[5] = 0;
+  flags[14] = 0;
 }
 
 }  // namespace
@@ -321,14 +333,18 @@ tm_status tm_run_tiles(tm_ctx *c, bool check) {
   TM_TRY(tm_ensure(c, &c->d_tile_cnt, c->cap_tile_cnt, nt, "tile sizes"));
   TM_TRY(tm_ensure(c, &c->d_tile_cell, c->cap_tile_cell, nt * TM_TILE, "tile ranks"));
   TM_TRY(tm_ensure(c, &c->d_nbr16, c->cap_nbr16, tm_slot_elems(nt * TM_TILE, c->nbr_alloc), "tile lists"));
-  k_zero_two<<<1, 1, 0, c->stream>>>(c->d_flags + 4);
-  k_tiles<<<(unsigned)nt, TM_TILE, 0, c->stream>>>(n, c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_tile_idx,
-                                                    c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->d_flags);
+  k_zero_tile_flags<<<1, 1, 0, c->stream>>>(c->d_flags);
+  k_tiles<<<(unsigned)nt, TM_TILE, 0, c->stream>>>(n, c->nbr_alloc, c->tile_window ? (uint32_t)WB : 0u, c->d_nbr,
+                                                    c->d_ncount, c->d_tile_idx, c->d_tile_cnt, c->d_tile_cell,
+                                                    c->d_nbr16, c->d_flags);
   TM_HIP(c, hipGetLastError());
   if (check) {
     TM_HIP(c, hipMemcpyAsync(c->h_flags + 4, c->d_flags + 4, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost,
                              c->stream));
+    TM_HIP(c, hipMemcpyAsync(c->h_flags + 14, c->d_flags + 14, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                             c->stream));
     TM_HIP(c, hipStreamSynchronize(c->stream));
+    c->tiles_over = c->h_flags[14];
     // records staged per tile: the largest tile plus the far stand-in, rounded
     // (at least TM_TILE: the pair kernel also parks a tile's sums there) ...
     int64_t lds = std::max<int64_t>(TM_TILE,
