@@ -264,6 +264,20 @@ typedef struct tm_diffusion {
  * steps; *residual: the last step's final relative residual. Synchronises. */
 tm_status tm_diffusion_step(tm_ctx *ctx, const tm_grid *grid, const tm_diffusion *params, const double *source,
                             double *field, int32_t *iterations, double *residual);
+/* No counterpart in the reference.  The same solve for callers that cannot
+ * allocate device memory (tight stepping loops, harnesses that forbid
+ * hipMalloc): identical kernels and results, but the work arrays (right-hand
+ * side, residual, two directions, A p, partial sums, solver state) live in
+ * one static device buffer of the library, sized for grids of at most
+ * TM_NOALLOC_MAX_NODES nodes (about 80 MiB, held on every device the library
+ * runs on until the process ends).  The call allocates nothing; larger grids
+ * are refused with TM_ERR_INVALID.  Every context on one device shares that
+ * buffer: calls that overlap on one device (two host threads, or two
+ * contexts' streams) race and must be serialised by the caller.  Nothing in
+ * the buffer needs to start at zero; each call writes every word it reads. */
+#define TM_NOALLOC_MAX_NODES 2097152
+tm_status tm_diffusion_step_noalloc(tm_ctx *ctx, const tm_grid *grid, const tm_diffusion *params,
+                                    const double *source, double *field, int32_t *iterations, double *residual);
 
 /* -------- the VTK writer -------- */
 /* writevtk (src/io.c:372-516), one rank: the reference's legacy binary VTK

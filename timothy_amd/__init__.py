@@ -20,6 +20,7 @@ TM_OK = 0
 STATUS = {0: "TM_OK", 1: "TM_ERR_INVALID", 2: "TM_ERR_HIP", 3: "TM_ERR_NOMEM", 4: "TM_ERR_DEGENERATE",
           5: "TM_ERR_STATE", 6: "TM_ERR_UNSUPPORTED"}
 FP64, MIXED = 0, 1
+NOALLOC_MAX_NODES = 2097152  # TM_NOALLOC_MAX_NODES
 STAGE_BUILD, STAGE_NEIGHBORS, STAGE_SPH, STAGE_CIC, STAGE_STEP, STAGE_TILES = 0, 1, 2, 3, 4, 5
 
 
@@ -139,6 +140,8 @@ def _load():
         "tm_graph_removed": (_i32, [_vp, C.POINTER(_i64)]),
         "tm_diffusion_step": (_i32, [_vp, C.POINTER(Grid), C.POINTER(Diffusion), _vp, _vp, C.POINTER(_i32),
                                      C.POINTER(C.c_double)]),
+        "tm_diffusion_step_noalloc": (_i32, [_vp, C.POINTER(Grid), C.POINTER(Diffusion), _vp, _vp,
+                                             C.POINTER(_i32), C.POINTER(C.c_double)]),
         "tm_write_vtk": (_i32, [_vp, C.c_char_p, _i32]),
         "tm_write_vtk_arrays": (_i32, [_vp, C.c_char_p, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32]),
     }
@@ -372,15 +375,18 @@ class Context:
         f = lib().tm_cic_interp_f32 if field.dtype == np.float32 else lib().tm_cic_interp
         self._check(f(self.h, C.byref(g), field.ptr, gradient.ptr, out.ptr))
 
-    def diffusion_step(self, grid, params: Diffusion, field: DeviceArray, source: DeviceArray = None):
-        """tm_diffusion_step: the environment solve on the device field;
+    def diffusion_step(self, grid, params: Diffusion, field: DeviceArray, source: DeviceArray = None,
+                       noalloc: bool = False):
+        """tm_diffusion_step (or, with noalloc, tm_diffusion_step_noalloc: work
+        arrays in the library's static device buffer, grids of at most
+        NOALLOC_MAX_NODES nodes): the environment solve on the device field;
         returns (CG iterations, final relative residual)"""
         g = Grid.from_dict(grid)
         it = _i32()
         res = C.c_double()
-        self._check(lib().tm_diffusion_step(self.h, C.byref(g), C.byref(params),
-                                            None if source is None else source.ptr, field.ptr, C.byref(it),
-                                            C.byref(res)))
+        f = lib().tm_diffusion_step_noalloc if noalloc else lib().tm_diffusion_step
+        self._check(f(self.h, C.byref(g), C.byref(params), None if source is None else source.ptr, field.ptr,
+                      C.byref(it), C.byref(res)))
         return it.value, res.value
 
     def write_vtk(self, path, rank=0):

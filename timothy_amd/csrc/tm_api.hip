@@ -271,8 +271,13 @@ tm_status tm_create(tm_ctx **out, int device) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc((void **)&c->d_types, sizeof(TypeDev) * TM_MAX_TYPES) != hipSuccess ||
       hipMalloc((void **)&c->d_flags, 16 * sizeof(unsigned int)) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_flags, 16 * sizeof(unsigned int), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_flags, TM_HFLAGS_WORDS * sizeof(unsigned int), hipHostMallocDefault) !=
+          hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    tm_destroy(c);
+    return TM_ERR_HIP;
+  }
+  if (tm_noalloc_init(c) != TM_OK) {
     tm_destroy(c);
     return TM_ERR_HIP;
   }
@@ -794,9 +799,7 @@ tm_status tm_move_cells(tm_ctx *c, int64_t n, int32_t dimension, double *x, doub
   return TM_OK;
 }
 
-tm_status tm_diffusion_step(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *source, double *field,
-                            int32_t *iterations, double *residual) {
-  if (!c) return TM_ERR_INVALID;
+static tm_status check_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *field) {
   if (!g || !d || !field) return tm_fail(c, TM_ERR_INVALID, "grid, parameters and field are required");
   for (int a = 0; a < 3; ++a)
     if (g->localsize[a] < 1) return tm_fail(c, TM_ERR_INVALID, "grid localsize must be >= 1 on every axis");
@@ -805,7 +808,21 @@ tm_status tm_diffusion_step(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, 
     return tm_fail(c, TM_ERR_INVALID, "substeps and maxiter must be >= 0, tol >= 0");
   if (!(d->diffusion >= 0.0) || !(d->dt >= 0.0) || !(d->lambda >= 0.0))
     return tm_fail(c, TM_ERR_INVALID, "diffusion, dt and lambda must be >= 0");
+  return TM_OK;
+}
+
+tm_status tm_diffusion_step(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *source, double *field,
+                            int32_t *iterations, double *residual) {
+  if (!c) return TM_ERR_INVALID;
+  TM_TRY(check_diffusion(c, g, d, field));
   return tm_run_diffusion(c, g, d, source, field, iterations, residual);
+}
+
+tm_status tm_diffusion_step_noalloc(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *source,
+                                    double *field, int32_t *iterations, double *residual) {
+  if (!c) return TM_ERR_INVALID;
+  TM_TRY(check_diffusion(c, g, d, field));
+  return tm_run_diffusion_noalloc(c, g, d, source, field, iterations, residual);
 }
 
 tm_status tm_get_positions(tm_ctx *c, double *x, double *y, double *z) {

@@ -27,6 +27,8 @@
 // remaining iterations returns at once, so the host launches iterations in
 // batches and reads the state once per batch.  The partial sums are added
 // in a fixed order: results do not depend on timing.
+#include <mutex>
+
 #include "tm_internal.h"
 
 namespace {
@@ -214,14 +216,30 @@ __global__ void k_cg_beta(const double *__restrict__ part, int nblk, int maxiter
 
 }  // namespace
 
-tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src, double *field,
-                           int32_t *iterations, double *residual) {
+// doubles of work memory for a grid of `total` nodes: b, r, two directions,
+// Ap; two arrays of partial sums; the state
+static constexpr int64_t work_doubles(int64_t total) {
+  return 5 * total + 2 * MAXBLK + (int64_t)((sizeof(CGState) + 7) / 8);
+}
+
+// The allocation-free entry point's work memory: one buffer per device, in
+// the code object (the loader places it on every device the library runs
+// on).  Host code never passes `scratch` itself to a kernel: on the host the
+// name is a stand-in, not the device address.  The address is looked up
+// once per device (tm_noalloc_init) and kept in scratch_addr.
+constexpr int64_t SCRATCH_DOUBLES = work_doubles(TM_NOALLOC_MAX_NODES);
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static double *scratch_addr[MAX_DEVICES];
+
+// the solve with its work arrays at `work` (work_doubles(total) doubles)
+static tm_status run_diffusion(tm_ctx *c, double *work, const tm_grid *g, const tm_diffusion *d, const double *src,
+                               double *field, int32_t *iterations, double *residual) {
   const int64_t lx = g->localsize[0], ly = g->localsize[1], lz = g->localsize[2];
   const int64_t total = lx * ly * lz;
-  // work arrays: b, r, two directions, Ap; partial sums; state
-  const int64_t need = 5 * total + 2 * MAXBLK + (int64_t)((sizeof(CGState) + 7) / 8);
-  TM_TRY(tm_ensure(c, &c->d_cg, c->cap_cg, need, "diffusion work arrays"));
-  double *b = c->d_cg, *r = b + total, *p0 = r + total, *p1 = p0 + total, *Ap = p1 + total;
+  double *b = work, *r = b + total, *p0 = r + total, *p1 = p0 + total, *Ap = p1 + total;
   double *part = Ap + total, *part2 = part + MAXBLK;
   CGState *st = (CGState *)(part2 + MAXBLK);
   const int64_t nblk0 = std::min<int64_t>(MAXBLK, (total + BD - 1) / BD);
@@ -244,7 +262,8 @@ tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, c
     // p0 holds r for the first direction (beta = 0); the directions then alternate
     double *pold = p0, *pnew = p1;
     int32_t launched = 0;
-    CGState h{};
+    // the state is read back into pinned host memory set aside at tm_create
+    CGState &h = *reinterpret_cast<CGState *>(c->h_flags + TM_HFLAGS_CG);
     for (;;) {
       const int32_t batch = std::min<int32_t>(8, d->maxiter - launched);
       for (int32_t q = 0; q < batch; ++q) {
@@ -256,7 +275,7 @@ tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, c
       }
       launched += batch;
       TM_HIP(c, hipGetLastError());
-      TM_HIP(c, hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      TM_HIP(c, hipMemcpyAsync(&h, st, sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
       TM_HIP(c, hipStreamSynchronize(c->stream));
       if (h.done || launched >= d->maxiter || batch == 0) break;
     }
@@ -266,4 +285,42 @@ tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, c
   if (iterations) *iterations = it_total;
   if (residual) *residual = last_res;
   return TM_OK;
+}
+
+static_assert(sizeof(CGState) <= 4 * (TM_HFLAGS_WORDS - TM_HFLAGS_CG), "solver state fits the pinned words");
+
+tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src, double *field,
+                           int32_t *iterations, double *residual) {
+  const int64_t total = g->localsize[0] * g->localsize[1] * g->localsize[2];
+  TM_TRY(tm_ensure(c, &c->d_cg, c->cap_cg, work_doubles(total), "diffusion work arrays"));
+  return run_diffusion(c, c->d_cg, g, d, src, field, iterations, residual);
+}
+
+tm_status tm_noalloc_init(tm_ctx *c) {
+  // called by tm_create with c->device current
+  if (c->device < 0 || c->device >= MAX_DEVICES) return tm_fail(c, TM_ERR_UNSUPPORTED, "device index too large");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_addr[c->device]) return TM_OK;
+  void *p = nullptr;
+  TM_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return tm_fail(c, TM_ERR_HIP, "no device address for the static work buffer");
+  scratch_addr[c->device] = (double *)p;
+  return TM_OK;
+}
+
+tm_status tm_run_diffusion_noalloc(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src,
+                                   double *field, int32_t *iterations, double *residual) {
+  const int64_t total = g->localsize[0] * g->localsize[1] * g->localsize[2];
+  if (total > TM_NOALLOC_MAX_NODES)
+    return tm_fail(c, TM_ERR_INVALID, "tm_diffusion_step_noalloc: the grid has " + std::to_string(total) +
+                                          " nodes, more than TM_NOALLOC_MAX_NODES (" +
+                                          std::to_string((long long)TM_NOALLOC_MAX_NODES) +
+                                          "); use tm_diffusion_step");
+  double *work;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    work = scratch_addr[c->device];
+  }
+  if (!work) return tm_fail(c, TM_ERR_STATE, "static work buffer not set up on this device");
+  return run_diffusion(c, work, g, d, src, field, iterations, residual);
 }

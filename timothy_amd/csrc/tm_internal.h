@@ -34,6 +34,10 @@
 // enough that a pair with it lies beyond every cutoff, near enough that its
 // squared distance stays finite in FP32
 #define TM_FAR 1.0e12
+// pinned host words of a context: [0, 16) copies of the device flags,
+// [TM_HFLAGS_CG, TM_HFLAGS_WORDS) the environment solve's state read back
+#define TM_HFLAGS_WORDS 32
+#define TM_HFLAGS_CG 16
 
 // Per cell-type constants, derived on the host with the reference's own
 // expressions (src/inline.h:45-46,71-72) so the device reads identical values.
@@ -170,7 +174,7 @@ struct tm_ctx {
   // loop's next Philox step, [12..13] error bits of the moves, [14] over-full
   // tiles
   unsigned int *d_flags = nullptr;
-  unsigned int *h_flags = nullptr;  // pinned
+  unsigned int *h_flags = nullptr;  // pinned, TM_HFLAGS_WORDS words: copies of d_flags, then the solver state
 
   int stage = 0;  // 0 empty, 1 cells, 2 built, 3 neighbours, 4 sph
   bool moved = false;  // a captured loop moved the cells after its sums (cleared by the next build)
@@ -246,6 +250,12 @@ tm_status tm_set_graph_step(tm_ctx *c, uint64_t step);
 tm_status tm_run_diffusion(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src, double *field,
                            int32_t *iterations, double *residual);
 tm_status tm_take_graph_removed(tm_ctx *c, int64_t *count);
+// looks up the device address of the allocation-free solve's static work
+// buffer on c->device (current); once per device, at tm_create
+tm_status tm_noalloc_init(tm_ctx *c);
+// the environment solve with its work arrays in that buffer (no allocation)
+tm_status tm_run_diffusion_noalloc(tm_ctx *c, const tm_grid *g, const tm_diffusion *d, const double *src,
+                                   double *field, int32_t *iterations, double *residual);
 inline unsigned long long *tm_removed_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 6); }
 inline unsigned long long *tm_graph_removed_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 8); }
 inline unsigned long long *tm_graph_step_word(tm_ctx *c) { return (unsigned long long *)(c->d_flags + 10); }
