@@ -259,9 +259,10 @@ def test_tile_hash_set_near_its_limit(tm, oracle_mod, ctx):
 def test_mixed_cells_with_one_neighbour_near_cutoff(tm, oracle_mod, ctx):
     """isolated pairs of cells at separation h (1 - eps) for eps from 1e-1 to
     1e-9 (each cell's only neighbour sits just inside the cutoff, where the
-    kernel terms vanish as eps^2 and eps^3), plus triplets with one close and
-    one near-cutoff neighbour: mixed force within 1e-5 of each cell's
-    sum_q |f_pq|, density within 1e-5, against the oracle"""
+    kernel terms vanish as eps^2 and eps^3), triplets with one close and one
+    near-cutoff neighbour, and isolated pairs 1e-1 to 1e-4 apart (the only
+    term is that of a very close pair): mixed force within 1e-5 of each
+    cell's sum_q |f_pq|, density within 1e-5, against the oracle"""
     O = oracle_mod
     rng = np.random.default_rng(17)
     pts = []
@@ -279,6 +280,14 @@ def test_mixed_cells_with_one_neighbour_near_cutoff(tm, oracle_mod, ctx):
                 w = rng.normal(size=3)
                 w /= np.linalg.norm(w)
                 pts.append(base + rng.uniform(0.3, 1.5) * w)
+    for dist in (1e-1, 1e-2, 1e-3, 1e-4):
+        for rep in range(20):
+            base = np.array([(k % 20) * 30.0, ((k // 20) % 20) * 30.0, (k // 400) * 30.0]) + rng.uniform(-3, 3, 3)
+            k += 1
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            pts.append(base)
+            pts.append(base + dist * v)
     X = np.array(pts)
     n = len(X)
     col = dict(n=n, x=X[:, 0].copy(), y=X[:, 1].copy(), z=X[:, 2].copy(), size=np.ones(n),

@@ -65,19 +65,25 @@ __device__ __forceinline__ void pair_fp64(const double4 &P, const double4 &Q, co
 // m = max(1 - 2u, 0), the two pieces of the cubic spline combine into
 //   W = 2 om^3 - m^3   and   w = -(dW/du)/6 = m^2 - om^2,
 // exact for both u <= 1/2 (where 2 om^3 - (1-2u)^3 = 1 - 6u^2 + 6u^3) and
-// u > 1/2 (m = 0), so no lane of a wave branches.  om comes from the FP64
-// difference of squares, om = (h^2 - d^2) / ((h + d) h), so a near-cutoff
-// term keeps its relative accuracy.  Beyond the cutoff both vanish exactly
-// (om = m = 0), which is what makes the far stand-in record a zero term.
-// Per-cell constants of the mixed pairs: h (FP32), h^2 (FP32 and FP64), -2/h.
+// u > 1/2 (m = 0), so no lane of a wave branches.  The force coefficient is
+// evaluated as
+//   w / d = -min(1/h, om/d) (om + m),
+// the same function (om - m = u below 1/2, m = 0 above, and u/d = 1/h)
+// without the cancellation of m^2 - om^2 as u -> 0, so a very close pair
+// keeps its relative accuracy.  om comes from the FP64 difference of squares,
+// om = (h^2 - d^2) / ((h + d) h), so a near-cutoff term keeps its relative
+// accuracy too.  Beyond the cutoff both vanish exactly (om = m = 0), which is
+// what makes the far stand-in record a zero term.
+// Per-cell constants of the mixed pairs: h (FP32), h^2 (FP32 and FP64), -2/h,
+// 1/h.
 struct MixedC {
-  float hf, hhf, m2hinv;
+  float hf, hhf, m2hinv, hinv;
   double hh;
 };
 
 __device__ __forceinline__ MixedC mixed_consts(const TypeDev &T) {
   const float hf = (float)T.h;
-  return MixedC{hf, hf * hf, -2.0f * T.hinv_f, T.h * T.h};
+  return MixedC{hf, hf * hf, -2.0f * T.hinv_f, T.hinv_f, T.h * T.h};
 }
 
 // One pair of the mixed form, added into the cell's sums: the separation
@@ -87,7 +93,7 @@ __device__ __forceinline__ MixedC mixed_consts(const TypeDev &T) {
 // cutoff, the terms there scaling as om^2 and om^3) and m = 1 - 2 d/h, both
 // clamped at 0 (W = 2 om^3 - m^3 covers both branches of the M4 kernel and
 // the far stand-in gives exact zeros); sw += size_q * W in FP32, and the
-// force coefficient (m^2 - om^2) / d times the FP64 separation added in FP64.
+// force coefficient w / d times the FP64 separation added in FP64.
 template <bool DIM3>
 __device__ __forceinline__ void pair_mixed(const double4 &P, double qx, double qy, double qz, float sq,
                                            const MixedC &K, float &sw, double &f0, double &f1, double &f2) {
@@ -100,7 +106,7 @@ __device__ __forceinline__ void pair_mixed(const double4 &P, double qx, double q
   const float m = fmaxf(__builtin_fmaf(d, K.m2hinv, 1.0f), 0.0f);
   const float om2 = om * om, m2 = m * m;
   sw = __builtin_fmaf(sq, __builtin_fmaf(om2 + om2, om, -(m2 * m)), sw);
-  const double coef = (double)((m2 - om2) * rinv);
+  const double coef = (double)(-fminf(K.hinv, om * rinv) * (om + m));
   f0 = __builtin_fma(coef, ex, f0);
   f1 = __builtin_fma(coef, ey, f1);
   if (DIM3) f2 = __builtin_fma(coef, ez, f2);
