@@ -182,7 +182,10 @@ tm_status tm_get_octree(tm_ctx *ctx, int32_t colony, void *nodes, int64_t *count
 /* -------- grid transfers (single colony; device arrays) -------- */
 /* weight_mode 0: reference weights by phase (5 -> 0, 0 -> 0.75, else 1.0);
  * weight_mode 1: cell volume 4/3*pi*size^3.  production is overwritten
- * (zeroed first, like patches_applycells2env). */
+ * (every node written, like patches_applycells2env's zeroing and sum).
+ * Each node holds the reference's terms summed in another order.  Grid node
+ * indices (loweridx .. upperidx) must lie in (-2^29, 2^29): TM_ERR_INVALID
+ * otherwise. */
 tm_status tm_cic_deposit(tm_ctx *ctx, const tm_grid *grid, int32_t weight_mode, double *production);
 /* gradient of a field with the reference's stencil (interior (f[i+1]-f[i-1])/res,
  * one-sided faces); FP64 or FP32 field */

@@ -9,6 +9,7 @@
 - Batched colonies in mixed precision, each colony against its own oracle run.
 - The tile build's hash set filled close to its limit.
 - Cells whose only neighbour sits just inside the cutoff, in mixed precision.
+- The 2M-cell workload's CIC deposit onto its own grid and a coarse one.
 """
 import numpy as np
 import pytest
@@ -307,3 +308,34 @@ def test_mixed_cells_with_one_neighbour_near_cutoff(tm, oracle_mod, ctx):
     fe, de = mixed_errors(col, off, nbr, m, ref)
     say(f"near-cutoff pairs n={n}: mixed force {fe:.2e} density {de:.2e}")
     assert fe <= 1e-5 and de <= 1e-5
+
+
+def test_deposit_2m_against_oracle(tm, oracle_mod, ctx):
+    """2M cells of that generator deposited onto 256^3 (its own grid: the
+    densest 16^3 tiles hold about 8000 cells and launch first) and onto 32^3
+    (eight tiles of about 250k cells each), against the oracle
+    (src/patches.c:156-234)"""
+    O = oracle_mod
+    col = gen.config(1)
+    ctx.upload_colony(col)
+    ctx.build()
+    oc = O.Colony.from_dict(col)
+    for G, modes in ((256, (0, 1)), (32, (0,))):
+        grid = col["grid"] if G == 256 else gen.bbox_grid(col, G)
+        # own cells of the densest 16^3 tile
+        ci = np.floor((np.stack([col["x"], col["y"], col["z"]], 1) - grid["lower"]) /
+                      float(grid["resolution"])).astype(np.int64)
+        t = np.maximum(ci, 0) // 16
+        nt = (G + 15) // 16
+        dense = np.bincount((t[:, 0] * nt + t[:, 1]) * nt + t[:, 2]).max()
+        assert dense >= 4096
+        for mode in modes:
+            prod = tm.DeviceArray(ctx, G ** 3, np.float64)
+            ctx.cic_deposit(grid, prod, mode)
+            got = prod.get()
+            prod.free()
+            ref = O.cic_deposit(oc, grid, mode)
+            assert np.array_equal(got != 0, ref != 0), (G, mode)
+            rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+            say(f"  deposit 2M onto {G}^3 mode {mode}: densest tile {dense} cells, max rel {rel.max():.2e}")
+            assert rel.max() <= 1e-12, (G, mode)

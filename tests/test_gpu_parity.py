@@ -370,6 +370,51 @@ def test_cic_and_field_gradient(tm, oracle_mod, ctx):
     assert np.array_equal(out.get().reshape(-1, 4), iref)
 
 
+def test_cic_deposit_grid_sequence(tm, oracle_mod, ctx):
+    """one context depositing onto a run of different grids: rank blocks with
+    non-zero lower indices and sizes that are not multiples of the tile edge,
+    thin blocks, a grid with more tiles than any before it (its per-tile
+    counts are fresh memory), back to a smaller one (counts left by the last
+    deposit) -- each against the oracle"""
+    O = oracle_mod
+    col = gen.config(0)
+    g0 = col["grid"]
+    ctx.upload_colony(col)
+    ctx.build()
+    oc = O.Colony.from_dict(col)
+
+    def block(lo, hi, lower=None):
+        lo, hi = np.array(lo, np.int64), np.array(hi, np.int64)
+        return dict(lower=g0["lower"] if lower is None else lower, resolution=g0["resolution"], loweridx=lo,
+                    upperidx=hi, localsize=hi - lo + 1)
+
+    res = float(g0["resolution"])
+    grids = [(block([0, 0, 0], [63, 63, 63]), 0),
+             (block([5, 17, 30], [45, 40, 63]), 0),
+             (block([0, 30, 0], [69, 32, 63]), 1),
+             (block([0, 0, 0], [87, 87, 87], lower=g0["lower"] - 12 * res), 1),
+             (block([31, 0, 9], [33, 63, 40]), 0),
+             (block([0, 0, 0], [63, 63, 63]), 1)]
+    for k, (grid, mode) in enumerate(grids):
+        ls = grid["localsize"]
+        prod = tm.DeviceArray(ctx, int(np.prod(ls)), np.float64)
+        ctx.cic_deposit(grid, prod, mode)
+        got = prod.get()
+        ref = O.cic_deposit(oc, grid, mode)
+        assert np.array_equal(got != 0, ref != 0), k
+        assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref)), k
+        say(f"  grid {k} ls={ls.tolist()} mode {mode}: {int((ref != 0).sum())} nodes, "
+            f"max rel {np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)):.2e}")
+        prod.free()
+    # node indices outside (-2^29, 2^29) are refused before any launch
+    far = block([-(1 << 29), 0, 0], [-(1 << 29) + 3, 3, 3])
+    prod = tm.DeviceArray(ctx, 64, np.float64)
+    with pytest.raises(tm.TMError) as e:
+        ctx.cic_deposit(far, prod, 0)
+    assert e.value.status == 1
+    prod.free()
+
+
 def test_batched_colonies_match_single(tm, oracle_mod, ctx):
     O = oracle_mod
     cols = gen.colonies_config3(ncol=12, scale=0.05)

@@ -11,8 +11,8 @@
 //   d = (x - cicCoord) / resolution, t = 1 - d, corner weight a*d + (1-a)*t
 // so corner indices match bit for bit and the interpolation (one lane per
 // cell, corners in the reference's ax, ay, az order) matches bit for bit.
-// The deposit sums the reference's terms into each node through LDS partial
-// sums and FP64 atomics: the same terms in a different order (<= ~1e-15
+// The deposit sums the reference's terms into each node owned by a grid
+// tile in LDS (k_dep_tile): the same terms in a different order (<= ~1e-15
 // relative for positive weights).
 #include "tm_internal.h"
 
@@ -54,112 +54,390 @@ __device__ __forceinline__ bool cic_inrange(const GridDev &g, const int64_t ci[3
          ci[1] + ay <= g.hi[1] && ci[2] + az <= g.hi[2];
 }
 
-__global__ void k_fill_f64(double *p, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) p[i] = 0.0;
+// Deposit by grid tile (owner computes; no global atomics, no separate
+// clearing).
+//
+// The grid's nodes are cut into tiles of DT^3 nodes; tile t owns the nodes
+// lo + DT t + [0, DT) per axis.  A cell with corner index ci reaches nodes
+// ci and ci + 1 per axis; it is keyed to the tile holding max(ci, lo) (the
+// clamp takes cells whose ci is lo - 1, which reach only node lo), so all
+// of its in-grid corners lie in that tile's nodes or, where its local index
+// is DT - 1, in the next tile's first plane.  Its layer bits mark those
+// axes.  Counting-sorting the cells by (tile, layer bits) puts, for each
+// tile, the cells that reach a neighbour's first plane in whole groups, so
+// tile t sums (into LDS) its own cells and, from each of its up to 7 lower
+// neighbours t - d, only the groups whose layer bits include d.  Every
+// owned node is then complete and written once with a plain store.
+// Within a node, terms are added in LDS atomic order (the reference adds
+// them in ascending cell order: the same terms, a different order).
+//
+// Integer work runs in 32 bits: the host admits grids whose node indices
+// lie in (-2^29, 2^29), and a cell whose scaled offset from the grid's lower
+// corner is 2^30 or more in magnitude reaches no node of such a grid.
+#ifndef TM_DEP_DT
+#define TM_DEP_DT 16
+#endif
+#ifndef TM_DEP_THREADS
+#define TM_DEP_THREADS 256
+#endif
+#ifndef TM_DEP_HEAVY
+#define TM_DEP_HEAVY 1024
+#endif
+constexpr int DT = TM_DEP_DT;              // tile edge in nodes (8 or 16)
+constexpr int DEP_T = TM_DEP_THREADS;      // threads per tile workgroup
+constexpr uint32_t DEP_HEAVY = TM_DEP_HEAVY;  // tiles with this many own cells are launched first
+static_assert(DT == 8 || DT == 16, "deposit tile edge");
+constexpr int DEP_IDX_LIMIT = 1 << 29;
+
+struct DepGrid {
+  double lower[3];
+  float res;
+  int lo[3], hi[3];  // node index range of this rank's block
+  int nt[3];         // tiles per axis
+};
+
+// LDS slot of owned node (lx, ly, lz): k fastest, with lz XOR-swizzled by
+// (lx, ly) so that the nodes of a small block of cells (what one wave's
+// Morton-consecutive cells reach) fall in different banks
+__device__ __forceinline__ int dep_slot(int lx, int ly, int lz) {
+  const int sw = DT == 16 ? (((ly >> 1) + 3 * lx) * 4) & 15 : ((lx + (ly >> 2)) * 4) & 7;
+  return (lx * DT + ly) * DT + (lz ^ sw);
 }
 
-// Deposit, one workgroup per 256 cells of the Morton order.  Those cells lie
-// in a small region, so their corners usually fit a box of at most DEP_LDS
-// nodes: the workgroup adds its terms into that box in LDS, then adds each
-// touched node to the grid with one global atomic (a few per cell instead of
-// eight).  A workgroup whose box is larger adds its terms to the grid
-// directly.
-constexpr int DEP_LDS = 4096;  // 32 KB of FP64 partial sums
+// One axis of the reference's cloud-in-cell set-up (src/patches.c:172-190):
+// corner index ci = trunc((p - lower) / res), corner coordinate
+// lower + (double)((float)ci * res), d = (p - corner) / res, and the weights
+// t = 1 - d of node ci and d of node ci + 1 (the reference's a d + (1 - a) t
+// for a = 0, 1).  False when the cell is too far from the grid to reach it.
+__device__ __forceinline__ bool dep_axis(const DepGrid &g, int a, double p, int &ci, double &w0, double &w1) {
+  const double q = (p - g.lower[a]) / (double)g.res;
+  if (!(fabs(q) < 1073741824.0)) return false;
+  ci = (int)q;
+  const float off = (float)ci * g.res;
+  const double corner = g.lower[a] + (double)off;
+  w1 = (p - corner) / (double)g.res;
+  w0 = 1.0 - w1;
+  return true;
+}
 
-__global__ __launch_bounds__(256) void k_deposit(const double4 *__restrict__ pos, const int32_t *__restrict__ phase_s,
-                                                 int64_t n, GridDev g, int weight_mode,
-                                                 double *__restrict__ prod) {
-  __shared__ double acc[DEP_LDS];
-  __shared__ int box[6];  // min x, y, z of the cells' first corners; max x, y, z
-  const int64_t i = tm_xcd_block() * 256 + threadIdx.x;
-  if (threadIdx.x < 3) box[threadIdx.x] = 0x7FFFFFFF;
-  else if (threadIdx.x < 6) box[threadIdx.x] = -0x7FFFFFFF;
-  double wt = 0.0;
-  bool dep = false;
-  double4 P = make_double4(0, 0, 0, 0);
-  if (i < n) {
-    P = pos[i];
-    if (weight_mode == 1) {
-      wt = ((4.0 / 3.0) * M_PI) * (P.w * P.w * P.w);
-      dep = true;
-    } else {
-      const int ph = phase_s[i];
-      dep = ph != 5;                 // necrotic: no deposit
-      wt = ph == 0 ? 0.75 : 1.0;     // G0: lower consumption
-    }
-  }
-  int64_t ci[3] = {0, 0, 0};
-  double d[3] = {0, 0, 0}, t[3] = {0, 0, 0};
-  if (dep) cic_setup(g, P.x, P.y, P.z, ci, d, t);
-  // the box only matters for corners in range: clamp to one node beyond it
-  int cl[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) cl[a] = (int)max(min(ci[a], g.hi[a] + 1), g.lo[a] - 1);
-  // the box over the depositing lanes: reduced across the wave first, then
-  // one LDS atomic per wave and bound (not one per lane on the same word)
-  int bmin[3], bmax[3];
+// the deposit weight (src/patches.c:213-227): volume mode, or 0.75 for G0
+// cells and 1 otherwise; necrotic cells (phase 5) deposit nothing
+__device__ __forceinline__ double dep_wt(const double4 &P, int ph, int weight_mode) {
+  return weight_mode == 1 ? ((4.0 / 3.0) * M_PI) * (P.w * P.w * P.w) : (ph == 0 ? 0.75 : 1.0);
+}
+
+// key of a cell: (tile, layer bits), or nkeys when it deposits nothing
+__device__ __forceinline__ uint32_t dep_key(const double4 &P, bool deposits, const DepGrid &g, uint32_t nkeys) {
+  if (!deposits) return nkeys;
+  const double pc[3] = {P.x, P.y, P.z};
+  uint32_t tc[3], layer = 0;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    bmin[a] = dep ? cl[a] : 0x7FFFFFFF;
-    bmax[a] = dep ? cl[a] : -0x7FFFFFFF;
+    int ci;
+    double w0, w1;
+    if (!dep_axis(g, a, pc[a], ci, w0, w1) || ci < g.lo[a] - 1 || ci > g.hi[a]) return nkeys;
+    const uint32_t c = (uint32_t)(max(ci, g.lo[a]) - g.lo[a]);
+    tc[a] = c / DT;
+    layer |= (c % DT == DT - 1 ? 1u : 0u) << a;
+  }
+  return ((tc[0] * (uint32_t)g.nt[1] + tc[1]) * (uint32_t)g.nt[2] + tc[2]) * 8u + layer;
+}
+
+// Lanes of the wave holding the same key are served by one atomic on
+// ctr[key] (consecutive cells of the Morton order share keys); returns this
+// lane's rank among the key's cells: ctr[key]'s old value plus its rank
+// among those lanes.
+__device__ __forceinline__ uint32_t wave_rank(uint32_t key, bool active, uint32_t *__restrict__ ctr) {
+  uint32_t mine = 0;
+  uint64_t left = __ballot(active);
+  while (left) {
+    const int lead = __builtin_ctzll(left);
+    const uint32_t lk = (uint32_t)__shfl((int)key, lead, 64);
+    const uint64_t same = __ballot(active && key == lk);
+    uint32_t base = 0;
+    if (__lane_id() == (unsigned)lead) base = atomicAdd(ctr + lk, (uint32_t)__builtin_popcountll(same));
+    base = (uint32_t)__shfl((int)base, lead, 64);
+    if (active && key == lk) mine = base + (uint32_t)__builtin_popcountll(same & tm_lanemask_lt());
+    left &= ~same;
+  }
+  return mine;
+}
+
+__global__ void k_fill_u32(uint32_t *p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Counting sort of the cells by key: count (each cell's rank within its key
+// from the count's atomics), exclusive scan, then each cell placed at its
+// key's offset plus its rank.  Within a key the order is that of the
+// atomics.  Each thread takes DEP_CU cells, 256 apart, all loads issued
+// first.
+constexpr int DEP_CU = 2;
+
+// kr[i] = key | rank << 32 (key nkeys: deposits nothing).  Block 0 also
+// clears the two launch-order counters ctr.
+__global__ __launch_bounds__(256) void k_dep_count(const double4 *__restrict__ pos,
+                                                   const int32_t *__restrict__ phase_s, int64_t n, DepGrid g,
+                                                   int weight_mode, uint32_t nkeys, uint64_t *__restrict__ kr,
+                                                   uint32_t *__restrict__ cnt, uint32_t *__restrict__ ctr) {
+  const int64_t i0 = (int64_t)blockIdx.x * (256 * DEP_CU) + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0u;
+  double4 P[DEP_CU];
+  int ph[DEP_CU];
+#pragma unroll
+  for (int u = 0; u < DEP_CU; ++u) {
+    const int64_t i = i0 + 256 * u;
+    P[u] = i < n ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    ph[u] = i < n && weight_mode != 1 ? phase_s[i] : 1;
   }
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
+  for (int u = 0; u < DEP_CU; ++u) {
+    const int64_t i = i0 + 256 * u;
+    const uint32_t k = i < n ? dep_key(P[u], ph[u] != 5, g, nkeys) : nkeys;
+    const uint32_t r = wave_rank(k, k < nkeys, cnt);
+    if (i < n) kr[i] = (uint64_t)k | (uint64_t)r << 32;
+  }
+}
+
+// Writes each keyed cell's record at its place in key order: (d_x, d_y, d_z,
+// weight), and its corner index relative to its tile's first node plus one
+// (0 .. DT per axis, 5 bits each), so that the tile pass needs no division.
+// The rank-0 cell of each key clears the key's count (the scan has read it),
+// so the counts are zero after the deposit.
+__global__ __launch_bounds__(256) void k_dep_place(const double4 *__restrict__ pos,
+                                                   const int32_t *__restrict__ phase_s,
+                                                   const uint64_t *__restrict__ kr, int64_t n, DepGrid g,
+                                                   int weight_mode, uint32_t nkeys, const uint32_t *__restrict__ offs,
+                                                   uint32_t *__restrict__ cnt, double4 *__restrict__ rec,
+                                                   uint32_t *__restrict__ loc) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t e = kr[i];
+  const uint32_t k = (uint32_t)e, r = (uint32_t)(e >> 32);
+  if (k >= nkeys) return;
+  const double4 P = pos[i];
+  const int ph = weight_mode != 1 ? phase_s[i] : 1;
+  const uint32_t slot = offs[k] + r;
+  if (r == 0) cnt[k] = 0u;
+  const uint32_t tile = k >> 3;
+  const int tl[3] = {(int)(tile / ((uint32_t)g.nt[2] * (uint32_t)g.nt[1])),
+                     (int)((tile / (uint32_t)g.nt[2]) % (uint32_t)g.nt[1]), (int)(tile % (uint32_t)g.nt[2])};
+  const double pc[3] = {P.x, P.y, P.z};
+  double d[3];
+  uint32_t packed = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      bmin[a] = min(bmin[a], __shfl_xor(bmin[a], o, 64));
-      bmax[a] = max(bmax[a], __shfl_xor(bmax[a], o, 64));
+  for (int a = 0; a < 3; ++a) {
+    int ci;
+    double w0;
+    (void)dep_axis(g, a, pc[a], ci, w0, d[a]);  // a keyed cell reaches the grid
+    packed |= (uint32_t)(ci - (g.lo[a] + DT * tl[a]) + 1) << (5 * a);
+  }
+  rec[slot] = make_double4(d[0], d[1], d[2], dep_wt(P, ph, weight_mode));
+  loc[slot] = packed;
+}
+
+// Exclusive scan of n words by one workgroup, out[n] the total, for the
+// key counts of grids of up to DEP_SCAN1_MAX keys (one launch instead of
+// three): chunks of SCAN1_C words staged through LDS (padded one word per
+// 32), the next chunk's loads issued before the current one is scanned.
+constexpr int SCAN1_T = 1024, SCAN1_E = 16, SCAN1_C = SCAN1_T * SCAN1_E;
+constexpr int64_t DEP_SCAN1_MAX = 1 << 16;
+__global__ __launch_bounds__(SCAN1_T) void k_dep_scan1(const uint32_t *__restrict__ in, uint32_t n,
+                                                       uint32_t *__restrict__ out) {
+  __shared__ uint32_t buf[SCAN1_C + SCAN1_C / 32];
+  __shared__ uint32_t wtot[SCAN1_T / 64];
+  const uint32_t t = threadIdx.x, lane = t & 63u, wv = t >> 6;
+  uint32_t nxt[SCAN1_E];
+#pragma unroll
+  for (int e = 0; e < SCAN1_E; ++e) {
+    const uint32_t i = e * SCAN1_T + t;
+    nxt[e] = i < n ? in[i] : 0u;
+  }
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < n; base += SCAN1_C) {
+#pragma unroll
+    for (int e = 0; e < SCAN1_E; ++e) {
+      const uint32_t i = e * SCAN1_T + t;
+      buf[i + (i >> 5)] = nxt[e];
     }
-  }
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(&box[a], bmin[a]);
-      atomicMax(&box[3 + a], bmax[a]);
+    for (int e = 0; e < SCAN1_E; ++e) {
+      const uint32_t i = base + SCAN1_C + e * SCAN1_T + t;
+      nxt[e] = i < n ? in[i] : 0u;
     }
-  }
-  __syncthreads();
-  const bool any = box[0] <= box[3];  // some lane deposits
-  const int b0 = box[0], b1 = box[1], b2 = box[2];
-  // extent of the corners ci and ci + 1
-  const int e0 = any ? box[3] - b0 + 2 : 0, e1 = any ? box[4] - b1 + 2 : 0, e2 = any ? box[5] - b2 + 2 : 0;
-  const bool local = any && (int64_t)e0 * e1 * e2 <= DEP_LDS;
-  if (local) {
-    const int vol = e0 * e1 * e2;
-    for (int k = threadIdx.x; k < vol; k += 256) acc[k] = 0.0;
+    __syncthreads();
+    uint32_t v[SCAN1_E], sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN1_E; ++k) {
+      const uint32_t j = SCAN1_E * t + k;
+      v[k] = buf[j + (j >> 5)];
+      sum += v[k];
+    }
+    uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= (uint32_t)o) inc += y;
+    }
+    if (lane == 63) wtot[wv] = inc;
+    __syncthreads();
+    uint32_t wb = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN1_T / 64; ++q) {
+      const uint32_t x = wtot[q];
+      if ((uint32_t)q < wv) wb += x;
+      tot += x;
+    }
+    uint32_t run = carry + wb + inc - sum;
+#pragma unroll
+    for (int k = 0; k < SCAN1_E; ++k) {
+      const uint32_t j = SCAN1_E * t + k;
+      buf[j + (j >> 5)] = run;
+      run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < SCAN1_E; ++e) {
+      const uint32_t i = e * SCAN1_T + t;
+      if (base + i < n) out[base + i] = buf[i + (i >> 5)];
+    }
+    carry += tot;
     __syncthreads();
   }
-  if (dep) {
+  if (t == 0) out[n] = carry;
+}
+
+// The runs of the sorted order a tile reads: its own run (all 8 layer
+// groups), then from each lower neighbour t - dd the groups whose layer bits
+// include dd, adjacent groups merged.  Entry: dd | first group << 3 | end
+// group << 6.
+constexpr int DEP_R = 14;
+__constant__ uint16_t dep_runs[DEP_R] = {
+    0 | 0 << 3 | 8 << 6, 1 | 1 << 3 | 2 << 6, 1 | 3 << 3 | 4 << 6, 1 | 5 << 3 | 6 << 6, 1 | 7 << 3 | 8 << 6,
+    2 | 2 << 3 | 4 << 6, 2 | 6 << 3 | 8 << 6, 3 | 3 << 3 | 4 << 6, 3 | 7 << 3 | 8 << 6, 4 | 4 << 3 | 8 << 6,
+    5 | 5 << 3 | 6 << 6, 5 | 7 << 3 | 8 << 6, 6 | 6 << 3 | 8 << 6, 7 | 7 << 3 | 8 << 6};
+
+// A launch slot of the tile pass: a record of DEP_SLOT words, [0] the tile,
+// [1 + r] the start of run r and [1 + DEP_R + r] the inclusive prefix of run
+// lengths through run r.
+constexpr int DEP_SLOT = 32;
+static_assert(1 + 2 * DEP_R <= DEP_SLOT, "slot record");
+
+// One thread per tile: its runs, and its slot in launch order.  Tiles with
+// at least DEP_HEAVY own cells take slots from the front, so that the
+// longest workgroups start first; the rest from the back.  ctr[0..1]
+// (front, back) start at zero.
+__global__ __launch_bounds__(256) void k_dep_order(const uint32_t *__restrict__ offs, DepGrid g, uint32_t ntiles,
+                                                   uint32_t *__restrict__ ctr, uint32_t *__restrict__ slots) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntiles) return;
+  const int tz = (int)(t % (uint32_t)g.nt[2]), ty = (int)((t / (uint32_t)g.nt[2]) % (uint32_t)g.nt[1]),
+            tx = (int)(t / ((uint32_t)g.nt[2] * (uint32_t)g.nt[1]));
+  uint32_t rb[DEP_R], rp[DEP_R], tot = 0;
 #pragma unroll
-    for (int ax = 0; ax < 2; ++ax)
-#pragma unroll
-      for (int ay = 0; ay < 2; ++ay)
-#pragma unroll
-        for (int az = 0; az < 2; ++az) {
-          if (!cic_inrange(g, ci, ax, ay, az)) continue;
-          const double val = wt * (ax * d[0] + (1 - ax) * t[0]) * (ay * d[1] + (1 - ay) * t[1]) *
-                             (az * d[2] + (1 - az) * t[2]);
-          if (local) {
-            const int lx = (int)ci[0] + ax - b0, ly = (int)ci[1] + ay - b1, lz = (int)ci[2] + az - b2;
-            atomicAdd(&acc[(lx * e1 + ly) * e2 + lz], val);
-          } else {
-            const int64_t gx = ci[0] + ax - g.lo[0], gy = ci[1] + ay - g.lo[1], gz = ci[2] + az - g.lo[2];
-            unsafeAtomicAdd(prod + (gx * g.ls[1] + gy) * g.ls[2] + gz, val);
-          }
-        }
+  for (int r = 0; r < DEP_R; ++r) {
+    const uint32_t e = dep_runs[r], dd = e & 7u;
+    const int nx = tx - (int)(dd & 1u), ny = ty - (int)((dd >> 1) & 1u), nz = tz - (int)((dd >> 2) & 1u);
+    uint32_t b = 0, len = 0;
+    if (nx >= 0 && ny >= 0 && nz >= 0) {
+      const uint32_t nk = (((uint32_t)nx * (uint32_t)g.nt[1] + (uint32_t)ny) * (uint32_t)g.nt[2] + (uint32_t)nz) * 8u;
+      b = offs[nk + ((e >> 3) & 7u)];
+      len = offs[nk + (e >> 6)] - b;
+    }
+    rb[r] = b;
+    tot += len;
+    rp[r] = tot;
   }
-  if (!local) return;
+  const uint32_t slot = rp[0] >= DEP_HEAVY ? atomicAdd(ctr, 1u) : ntiles - 1 - atomicAdd(ctr + 1, 1u);
+  uint32_t *w = slots + (size_t)slot * DEP_SLOT;
+  w[0] = t;
+#pragma unroll
+  for (int r = 0; r < DEP_R; ++r) {
+    w[1 + r] = rb[r];
+    w[1 + DEP_R + r] = rp[r];
+  }
+}
+
+// adds the corners of a cell that fall in the tile's owned in-grid nodes:
+// local corner index l (relative to the tile's first node) per axis, node
+// l + o owned when 0 <= l + o <= vhi; weights t = 1 - d and d as the
+// reference forms them, multiplied ((wt wx) wy) wz (src/patches.c:172-227)
+__device__ __forceinline__ void dep_cell(const double4 &R, const int l[3], const int vhi[3], double *acc) {
+  const double w[3][2] = {{1.0 - R.x, R.x}, {1.0 - R.y, R.y}, {1.0 - R.z, R.z}};
+#pragma unroll
+  for (int ax = 0; ax < 2; ++ax) {
+    if (l[0] + ax < 0 || l[0] + ax > vhi[0]) continue;
+    const double wx = R.w * w[0][ax];
+#pragma unroll
+    for (int ay = 0; ay < 2; ++ay) {
+      if (l[1] + ay < 0 || l[1] + ay > vhi[1]) continue;
+      const double wxy = wx * w[1][ay];
+#pragma unroll
+      for (int az = 0; az < 2; ++az) {
+        if (l[2] + az < 0 || l[2] + az > vhi[2]) continue;
+        atomicAdd(&acc[dep_slot(l[0] + ax, l[1] + ay, l[2] + az)], wxy * w[2][az]);
+      }
+    }
+  }
+}
+
+constexpr int DEP_U = 4;  // cells per thread and round in k_dep_tile
+
+// One workgroup per tile, in launch-slot order.  The tile's runs are laid
+// end to end as one index space that every thread strides through, DEP_U
+// cells per round with all loads issued first.
+__global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ rec,
+                                                  const uint32_t *__restrict__ loc,
+                                                  const uint32_t *__restrict__ slots, DepGrid g, int64_t ls1,
+                                                  int64_t ls2, double *__restrict__ prod) {
+  __shared__ double acc[DT * DT * DT];
+  __shared__ uint32_t w[DEP_SLOT];  // the slot record; w[1 + r] run starts, w[1 + DEP_R + r] prefix
+  if (threadIdx.x < DEP_SLOT) w[threadIdx.x] = slots[(size_t)blockIdx.x * DEP_SLOT + threadIdx.x];
+  for (int k = threadIdx.x; k < DT * DT * DT; k += DEP_T) acc[k] = 0.0;
   __syncthreads();
-  const int vol = e0 * e1 * e2;
-  for (int k = threadIdx.x; k < vol; k += 256) {
-    const double v = acc[k];
-    if (v == 0.0) continue;  // untouched (only in-range corners were added)
-    const int lz = k % e2, ly = (k / e2) % e1, lx = k / (e1 * e2);
-    const int64_t gx = b0 + lx - g.lo[0], gy = b1 + ly - g.lo[1], gz = b2 + lz - g.lo[2];
-    unsafeAtomicAdd(prod + (gx * g.ls[1] + gy) * g.ls[2] + gz, v);
+  const uint32_t tile = w[0];
+  const int tz = (int)(tile % (uint32_t)g.nt[2]), ty = (int)((tile / (uint32_t)g.nt[2]) % (uint32_t)g.nt[1]),
+            tx = (int)(tile / ((uint32_t)g.nt[2] * (uint32_t)g.nt[1]));
+  const int T[3] = {g.lo[0] + DT * tx, g.lo[1] + DT * ty, g.lo[2] + DT * tz};
+  const int vhi[3] = {min(g.hi[0] - T[0], DT - 1), min(g.hi[1] - T[1], DT - 1), min(g.hi[2] - T[2], DT - 1)};
+  const uint32_t total = w[2 * DEP_R];
+  for (uint32_t v0 = 0; v0 < total; v0 += DEP_U * DEP_T) {
+    uint32_t p[DEP_U], dd[DEP_U];
+#pragma unroll
+    for (int u = 0; u < DEP_U; ++u) {
+      const uint32_t v = v0 + DEP_T * u + threadIdx.x;
+      p[u] = UINT32_MAX;
+      dd[u] = 0;
+      if (v < total) {
+        int r = 0;
+        while (w[1 + DEP_R + r] <= v) ++r;  // ends at r < DEP_R since v < total
+        p[u] = w[1 + r] + (v - (r == 0 ? 0u : w[DEP_R + r]));
+        dd[u] = dep_runs[r] & 7u;
+      }
+    }
+    double4 R[DEP_U];
+    uint32_t L[DEP_U];
+#pragma unroll
+    for (int u = 0; u < DEP_U; ++u) {
+      R[u] = p[u] != UINT32_MAX ? rec[p[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
+      L[u] = p[u] != UINT32_MAX ? loc[p[u]] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < DEP_U; ++u) {
+      if (p[u] == UINT32_MAX) continue;
+      int l[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) l[a] = (int)((L[u] >> (5 * a)) & 31u) - 1 - (int)((dd[u] >> a) & 1u) * DT;
+      dep_cell(R[u], l, vhi, acc);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < DT * DT * DT; k += DEP_T) {
+    const int lx = k / (DT * DT), ly = (k / DT) % DT, lz = k % DT;
+    if (lx > vhi[0] || ly > vhi[1] || lz > vhi[2]) continue;
+    prod[((int64_t)(T[0] + lx - g.lo[0]) * ls1 + (T[1] + ly - g.lo[1])) * ls2 + (T[2] + lz - g.lo[2])] =
+        acc[dep_slot(lx, ly, lz)];
   }
 }
 
@@ -226,12 +504,55 @@ __global__ __launch_bounds__(256) void k_gradient(const F *__restrict__ f, F *__
 }  // namespace
 
 tm_status tm_run_deposit(tm_ctx *c, const tm_grid *grid, int weight_mode, double *prod) {
-  GridDev g = to_dev(grid);
-  const int64_t total = g.ls[0] * g.ls[1] * g.ls[2];
-  k_fill_f64<<<(unsigned)std::min<int64_t>(tm_blocks(total, 256), 65536), 256, 0, c->stream>>>(prod, total);
-  if (c->n > 0)
-    k_deposit<<<tm_blocks(c->n, 256), 256, 0, c->stream>>>(c->d_pos, c->d_phase_s, c->n, g, weight_mode, prod);
+  DepGrid g;
+  for (int a = 0; a < 3; ++a) {
+    if (grid->loweridx[a] <= -DEP_IDX_LIMIT || grid->upperidx[a] >= DEP_IDX_LIMIT)
+      return tm_fail(c, TM_ERR_INVALID, "grid node indices must lie in (-2^29, 2^29) for the deposit");
+    g.lower[a] = grid->lower[a];
+    g.lo[a] = (int)grid->loweridx[a];
+    g.hi[a] = (int)grid->upperidx[a];
+    g.nt[a] = (int)((grid->localsize[a] + DT - 1) / DT);
+  }
+  g.res = grid->resolution;
+  const int64_t ntiles = (int64_t)g.nt[0] * g.nt[1] * g.nt[2];
+  const int64_t nkeys = ntiles * 8;  // the key nkeys marks a cell that deposits nothing
+  if (nkeys >= (int64_t)UINT32_MAX) return tm_fail(c, TM_ERR_INVALID, "grid too large for the deposit's tiles");
+  const int64_t n = c->n;
+  if ((uint64_t)ntiles * DEP_SLOT >= UINT32_MAX) return tm_fail(c, TM_ERR_INVALID, "grid too large for the deposit's tiles");
+  // per key a count (nkeys + 1, the last always 0), two order counters, per
+  // key an exclusive offset (nkeys + 1: offs[nkeys] is the number of keyed
+  // cells), the tiles' slot records
+  const int64_t cap0 = c->cap_drange;
+  TM_TRY(tm_ensure(c, &c->d_drange, c->cap_drange, 2 * nkeys + 4 + DEP_SLOT * ntiles, "deposit offsets"));
+  if (c->cap_drange != cap0) c->dep_zero = 0;
+  uint32_t *cnt = c->d_drange, *ctr = cnt + nkeys + 1, *offs = ctr + 2, *slots = offs + nkeys + 1;
+  // counts left at zero by the last deposit need no clearing; the counters
+  // are cleared by k_dep_count
+  if (c->dep_zero < nkeys + 1 || n == 0) {
+    k_fill_u32<<<(unsigned)std::min<int64_t>(tm_blocks(nkeys + 3, 256), 4096), 256, 0, c->stream>>>(cnt, nkeys + 3);
+  }
+  c->dep_zero = 0;
+  if (n > 0) {
+    TM_TRY(tm_ensure(c, &c->d_dkr, c->cap_dkr, n, "deposit keys"));
+    TM_TRY(tm_ensure(c, &c->d_drec, c->cap_drec, n, "deposit records"));
+    TM_TRY(tm_ensure(c, &c->d_dloc, c->cap_dloc, n, "deposit corner indices"));
+    k_dep_count<<<tm_blocks(n, 256 * DEP_CU), 256, 0, c->stream>>>(c->d_pos, c->d_phase_s, n, g, weight_mode,
+                                                          (uint32_t)nkeys, c->d_dkr, cnt, ctr);
+  }
+  if (nkeys <= DEP_SCAN1_MAX) {
+    k_dep_scan1<<<1, SCAN1_T, 0, c->stream>>>(cnt, (uint32_t)nkeys, offs);
+  } else {
+    TM_TRY(tm_scan_u32(c, cnt, offs, nkeys + 1));
+  }
+  if (n > 0)
+    k_dep_place<<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_pos, c->d_phase_s, c->d_dkr, n, g, weight_mode,
+                                                          (uint32_t)nkeys, offs, cnt, c->d_drec, c->d_dloc);
+  k_dep_order<<<tm_blocks(ntiles, 256), 256, 0, c->stream>>>(offs, g, (uint32_t)ntiles, ctr, slots);
+  // with no cells every offset is 0 and no record is read
+  k_dep_tile<<<(unsigned)ntiles, DEP_T, 0, c->stream>>>(c->d_drec, c->d_dloc, slots, g, grid->localsize[1],
+                                                        grid->localsize[2], prod);
   TM_HIP(c, hipGetLastError());
+  c->dep_zero = nkeys + 1;
   return TM_OK;
 }
 
