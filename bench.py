@@ -3,10 +3,11 @@
 Default workload (the metric's): one colony of 2,000,000 cells (radius 1.0,
 cutoff h = 2.0) drawn from a seeded Gaussian spheroid (sigma = 0.35 * 146.6,
 z axis * 0.7), held in HBM.  Untimed: Morton sort, radix tree, neighbour
-lists.  Timed: K launches of the SPH density+force stage (density, v and
-forces over the neighbour lists), mixed precision and plain FP64, each
-bracketed by HIP events on the library's stream and synchronised on both
-sides.  `value` is the mixed form's pair-interactions/s (P / t_sph, P =
+lists (with v and mindist, which the neighbour walk produces), LDS tiles,
+and first tm_autotune (mixed; --no-autotune keeps the default shapes).
+Timed: K launches of the SPH density+force stage (density and forces over
+the neighbour lists), mixed precision and plain FP64, each bracketed by HIP
+events on the library's stream and synchronised on both sides.  `value` is the mixed form's pair-interactions/s (P / t_sph, P =
 ordered in-cutoff pairs); the FP64 drop-in, the full-step time (sort + tree +
 neighbour lists + SPH + CIC deposit to a 256^3 FP64 grid), the resident
 mechanics loop (a captured graph whose every replay sorts, builds, lists,
@@ -58,12 +59,23 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads for the CPU baseline (0 = all)")
     ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default shapes instead of running tm_autotune (mixed) first")
     return ap.parse_args()
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def autotune(tm, ctx, a):
+    """tm_autotune (mixed) before timing: the shapes it keeps and its trial count"""
+    if not a.autotune:
+        return None
+    t0 = time.perf_counter()
+    r = ctx.autotune(tm.MIXED)
+    log(r["text"].splitlines()[-1])
+    return {"chosen": r["chosen"], "trials": len(r["trials"]), "seconds": round(time.perf_counter() - t0, 3)}
 
 
 def traffic_from_profiles():
@@ -279,7 +291,7 @@ def run_single(a, tm, gen):
     ctx.step(tm.FP64)
     P, maxnbr = ctx.pair_count()
     ref = ctx.results()  # FP64 drop-in, bit-exact with the CPU path
-    tune = ctx.autotune(tm.MIXED) if a.autotune else None
+    tune = autotune(tm, ctx, a)
     log(f"single colony n={n} P={P} max_nbr={maxnbr}")
     out, B = time_sph(tm, ctx, a, n, P)
     err = None
@@ -387,7 +399,7 @@ def run_batched(a, tm, gen):
     P, maxnbr = ctx.pair_count()
     ref = ctx.results()
     log(f"batched: {len(cols)} colonies, n={n} P={P} max_nbr={maxnbr}")
-    tune = ctx.autotune(tm.MIXED) if a.autotune else None
+    tune = autotune(tm, ctx, a)
     out, B = time_sph(tm, ctx, a, n, P)
     # graph-captured step: build + neighbour lists + mixed SPH for all colonies
     ctx.graph_capture(tm.MIXED)
@@ -478,7 +490,7 @@ def run_large(a, tm, gen):
     P, maxnbr = ctx.pair_count()
     ref = ctx.results()
     log(f"large: P={P} max_nbr={maxnbr}")
-    tune = ctx.autotune(tm.MIXED) if a.autotune else None
+    tune = autotune(tm, ctx, a)
     out, B = time_sph(tm, ctx, a, n, P)
     err = None
     if "mixed" in out:

@@ -119,7 +119,10 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * decide; results are identical; default 1), "tile_window" (1 = the tile
  * build orders the distinct neighbours within 8192 cells of a tile's middle
  * by bitmap and hashes only the rest; 0 = hashes all of them; results are
- * identical; default 1); read-only "nbr_alloc" (slots per cell of the
+ * identical; default 1), "tile_width" cells per LDS tile (128, 256, 512;
+ * default 256; takes effect at the next tm_neighbors; results are
+ * identical); a change of "nbr_cap" or "tile_width" retires a captured graph;
+ * read-only "nbr_alloc" (slots per cell of the
  * current lists), "tile_lds" (records staged per tile, 0 without tiles) and
  * "tiles_over" (tiles with more than 1536 distinct neighbours, which take
  * the 32-bit lists, at the last tm_neighbors) */
@@ -324,8 +327,14 @@ tm_status tm_graph_launch(tm_ctx *ctx);
 tm_status tm_graph_destroy(tm_ctx *ctx);
 
 /* -------- autotuner -------- */
-/* times candidate launch shapes for the current colonies and keeps the fastest;
- * writes a short report into buf (may be NULL) */
+/* Times candidate shapes for the current colonies and keeps the fastest: the
+ * walk (nbr_block x walk_window, then walk_f32), then with LDS tiles (3-D)
+ * tile_width (128, 256, 512) x sph_unroll timed as tile build + pair kernel,
+ * else sph_block x sph_unroll (x sph_lanes, mixed).  No candidate changes a
+ * result.  Leaves the lists and tiles built with the chosen shapes.  Report
+ * into buf (may be NULL): one line "trial <stage> key=value ... ms=<t>" per
+ * candidate, then "chosen key=value ...".  Replaces the reference's only
+ * knob, the OpenMP schedule of the walks (src/potential.c:90-93). */
 tm_status tm_autotune(tm_ctx *ctx, tm_precision precision, char *buf, size_t buflen);
 
 #ifdef __cplusplus

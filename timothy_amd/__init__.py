@@ -442,9 +442,41 @@ class Context:
         self._check(lib().tm_graph_destroy(self.h))
 
     def autotune(self, precision=MIXED):
-        buf = C.create_string_buffer(4096)
-        self._check(lib().tm_autotune(self.h, precision, buf, 4096))
-        return buf.value.decode()
+        """runs tm_autotune; returns {"trials": [...], "chosen": {...}, "text": report}
+        (values parsed as int or float where they are numbers)"""
+        buf = C.create_string_buffer(16384)
+        self._check(lib().tm_autotune(self.h, precision, buf, 16384))
+        return parse_autotune(buf.value.decode())
+
+
+def parse_autotune(text):
+    """the tm_autotune report as {"trials": [{"stage": .., key: value, ..}],
+    "chosen": {key: value}, "text": text}"""
+    def fields(words):
+        d = {}
+        for w in words:
+            k, _, v = w.partition("=")
+            try:
+                d[k] = int(v)
+            except ValueError:
+                try:
+                    d[k] = float(v)
+                except ValueError:
+                    d[k] = v
+        return d
+
+    out = {"trials": [], "chosen": None, "text": text}
+    for line in text.splitlines():
+        w = line.split()
+        if not w:
+            continue
+        if w[0] == "trial":
+            out["trials"].append(dict(stage=w[1], **fields(w[2:])))
+        elif w[0] == "chosen":
+            out["chosen"] = fields(w[1:])
+        else:
+            raise ValueError(f"unparsed autotune line: {line!r}")
+    return out
 
 
 def header_functions(path=None):

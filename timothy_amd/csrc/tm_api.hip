@@ -406,8 +406,12 @@ tm_status tm_set_celltypes(tm_ctx *c, const tm_celltype *types, int32_t ntypes, 
 
 tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
   if (!c || !key) return TM_ERR_INVALID;
+  // a captured graph holds the buffers of the lists and tiles; a parameter
+  // that changes their sizes retires it (tm_graph_launch then reports that
+  // there is none until the next capture)
   if (!strcmp(key, "nbr_cap")) {
     if (v < 1 || v > 4096) return tm_fail(c, TM_ERR_INVALID, "nbr_cap in [1, 4096]");
+    if (((v + 7) & ~7ll) != c->nbr_cap) tm_graph_destroy(c);
     c->nbr_cap = (v + 7) & ~7ll;  // lists are read in chunks of 8 slots
   } else if (!strcmp(key, "sph_block")) {
     if (v != 64 && v != 128 && v != 256) return tm_fail(c, TM_ERR_INVALID, "sph_block in {64,128,256}");
@@ -432,6 +436,10 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->walk_f32 = v ? 1 : 0;
   } else if (!strcmp(key, "tile_window")) {
     c->tile_window = v ? 1 : 0;
+  } else if (!strcmp(key, "tile_width")) {
+    if (v != 128 && v != 256 && v != 512) return tm_fail(c, TM_ERR_INVALID, "tile_width in {128,256,512}");
+    if (v != c->tile_width) tm_graph_destroy(c);
+    c->tile_width = (int)v;
   } else if (!strcmp(key, "walk_window")) {
     if (v != 0 && v != 512 && v != 1024 && v != 2048)
       return tm_fail(c, TM_ERR_INVALID, "walk_window in {0,512,1024,2048}");
@@ -456,6 +464,7 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "walk_window")) *v = c->walk_window;
   else if (!strcmp(key, "walk_f32")) *v = c->walk_f32;
   else if (!strcmp(key, "tile_window")) *v = c->tile_window;
+  else if (!strcmp(key, "tile_width")) *v = c->tile_width;
   else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
   else if (!strcmp(key, "tiles_over")) *v = c->tiles_over;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
@@ -890,11 +899,14 @@ tm_status tm_graph_launch(tm_ctx *c) {
   return TM_OK;
 }
 
+// Report: one line per trial, "trial <stage> key=value ... ms=<time>", then
+// "chosen key=value ...".  Every candidate gives the same results (the FP64
+// sums keep each list's order whatever the shapes).
 tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   if (!c) return TM_ERR_INVALID;
   if (c->stage < 3) TM_TRY(tm_step(c, p));
   std::string rep;
-  char line[160];
+  char line[200];
   // neighbour walk workgroup and the tree window it stages in LDS
   int best_nb = c->nbr_block, best_win = c->walk_window;
   double best_t = 1e30;
@@ -904,7 +916,7 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
     c->walk_window = sh[1];
     double ms = 0;
     TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 3, &ms));
-    snprintf(line, sizeof(line), "nbr_block=%d walk_window=%d %.4f ms\n", sh[0], sh[1], ms);
+    snprintf(line, sizeof(line), "trial neighbors nbr_block=%d walk_window=%d ms=%.4f\n", sh[0], sh[1], ms);
     rep += line;
     if (ms < best_t) {
       best_t = ms;
@@ -921,43 +933,75 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
     for (int k = 0; k < 2; ++k) {
       c->walk_f32 = k == 0 ? f0 : !f0;
       TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 10, &t[k]));
-      snprintf(line, sizeof(line), "walk_f32=%d %.4f ms\n", c->walk_f32, t[k]);
+      snprintf(line, sizeof(line), "trial neighbors walk_f32=%d ms=%.4f\n", c->walk_f32, t[k]);
       rep += line;
     }
     c->walk_f32 = t[1] < t[0] ? !f0 : f0;
   }
-  TM_TRY(tm_run_neighbors(c, true));
-  // pair kernel: workgroup size x neighbours fetched per trip
-  int best_sb = c->sph_block;
   int &unroll = (p == TM_MIXED) ? c->sph_unroll_mixed : c->sph_unroll_fp64;
-  int best_u = unroll, best_l = c->sph_lanes;
-  best_t = 1e30;
-  const std::vector<int> lanes = (p == TM_MIXED) ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1};
-  for (int l : lanes) {
-    for (int bs : {64, 128, 256}) {
-      for (int u : {1, 2, 4, 8}) {
-        c->sph_block = bs;
+  const bool tiles = c->sph_tiles && c->dim == 3;
+  if (tiles) {
+    // tile width x neighbours fetched per trip, timed as tile build + pairs
+    const int w0 = c->tile_width;
+    int best_w = w0, best_u = unroll;
+    best_t = 1e30;
+    for (int w : {128, 256, 512}) {
+      c->tile_width = w;
+      TM_TRY(tm_run_neighbors(c, true));  // lists, and tiles of this width sized for LDS
+      double tt = 0;
+      TM_TRY(tm_time_stage(c, TM_STAGE_TILES, p, 1, 5, &tt));
+      for (int u : {2, 4, 8}) {
         unroll = u;
-        if (p == TM_MIXED) c->sph_lanes = l;
         double ms = 0;
         TM_TRY(tm_time_stage(c, TM_STAGE_SPH, p, 2, 10, &ms));
-        snprintf(line, sizeof(line), "lanes=%d sph_block=%d unroll=%d %.4f ms\n", l, bs, u, ms);
+        snprintf(line, sizeof(line), "trial tiles_sph tile_width=%d unroll=%d tiles_ms=%.4f sph_ms=%.4f ms=%.4f\n", w,
+                 u, tt, ms, tt + ms);
         rep += line;
-        if (ms < best_t) {
-          best_t = ms;
-          best_sb = bs;
+        if (tt + ms < best_t) {
+          best_t = tt + ms;
+          best_w = w;
           best_u = u;
-          best_l = l;
         }
       }
     }
+    if (best_w != w0) tm_graph_destroy(c);  // a captured graph holds tiles of the old width
+    c->tile_width = best_w;
+    unroll = best_u;
+    TM_TRY(tm_run_neighbors(c, true));
+  } else {
+    // direct pair kernel: workgroup size x neighbours per trip (x lanes per cell)
+    TM_TRY(tm_run_neighbors(c, true));
+    int best_sb = c->sph_block, best_u = unroll, best_l = c->sph_lanes;
+    best_t = 1e30;
+    const std::vector<int> lanes = (p == TM_MIXED) ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1};
+    for (int l : lanes) {
+      for (int bs : {64, 128, 256}) {
+        for (int u : {1, 2, 4, 8}) {
+          c->sph_block = bs;
+          unroll = u;
+          if (p == TM_MIXED) c->sph_lanes = l;
+          double ms = 0;
+          TM_TRY(tm_time_stage(c, TM_STAGE_SPH, p, 2, 10, &ms));
+          snprintf(line, sizeof(line), "trial sph lanes=%d sph_block=%d unroll=%d ms=%.4f\n", l, bs, u, ms);
+          rep += line;
+          if (ms < best_t) {
+            best_t = ms;
+            best_sb = bs;
+            best_u = u;
+            best_l = l;
+          }
+        }
+      }
+    }
+    c->sph_block = best_sb;
+    unroll = best_u;
+    if (p == TM_MIXED) c->sph_lanes = best_l;
   }
-  c->sph_block = best_sb;
-  unroll = best_u;
-  if (p == TM_MIXED) c->sph_lanes = best_l;
-  snprintf(line, sizeof(line), "chosen nbr_block=%d walk_window=%d walk_f32=%d sph_block=%d unroll=%d lanes=%d (%s)\n",
-           best_nb, best_win, c->walk_f32, best_sb, best_u, p == TM_MIXED ? best_l : 1,
-           p == TM_MIXED ? "mixed" : "fp64");
+  snprintf(line, sizeof(line),
+           "chosen precision=%s nbr_block=%d walk_window=%d walk_f32=%d sph_tiles=%d tile_width=%d sph_block=%d "
+           "unroll=%d lanes=%d\n",
+           p == TM_MIXED ? "mixed" : "fp64", c->nbr_block, c->walk_window, c->walk_f32, tiles ? 1 : 0, c->tile_width,
+           c->sph_block, unroll, c->sph_lanes);
   rep += line;
   if (buf && buflen) {
     strncpy(buf, rep.c_str(), buflen - 1);

@@ -17,8 +17,9 @@
 #define TM_FLAG_CTYPE 4
 #define TM_FLAG_WALK 8
 #define TM_BBOX_CHUNK 8192
-// LDS tiles: the pair kernels take cells in tiles of TM_TILE consecutive
-// (Morton-order) cells; the distinct neighbours of a tile, at most
+// LDS tiles: the pair kernels take cells in tiles of tile_width (128, 256 or
+// 512; TM_TILE by default) consecutive (Morton-order) cells; the distinct
+// neighbours of a tile, at most
 // TM_TILE_MAX, are staged once in LDS and the lists hold 16-bit tile-local
 // indices.  A tile with more distinct neighbours is marked TM_TILE_OVER and
 // read through the 32-bit lists.
@@ -152,8 +153,11 @@ struct tm_ctx {
   int64_t tiles_over = 0;  // over-full tiles at the last checked tile build
   uint32_t *d_tile_idx = nullptr;   // per tile: distinct neighbours, ascending (TM_TILE_MAX slots)
   uint32_t *d_tile_cnt = nullptr;   // per tile: their number, or TM_TILE_OVER
-  uint8_t *d_tile_cell = nullptr;   // per tile: cell offset of each rank (longest list first)
-  uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_nbr16_at(g*TM_TILE + r, k, nbr_alloc)
+  uint16_t *d_tile_cell = nullptr;  // per tile: cell offset of each rank (longest list first)
+  uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_nbr16_at(g*TS + r, k, nbr_alloc)
+  int tile_width = TM_TILE;         // cells per tile for the next tile build: 128, 256 or 512
+  int tiles_ts = 0;                 // cells per tile of the tiles as built (0: none)
+  int tile_lds_ts = 0;              // the tile width tile_lds was sized for
   int64_t cap_tile_idx = 0, cap_tile_cnt = 0, cap_tile_cell = 0, cap_nbr16 = 0;
 
   // environment solve: right-hand side, residual, two directions, A p, partial sums, state

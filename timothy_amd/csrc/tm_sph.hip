@@ -316,7 +316,7 @@ __device__ __forceinline__ double stage_size(double size) {
   else return size;
 }
 
-// Pair kernel over LDS tiles: one workgroup of TM_TILE lanes per tile.  The
+// Pair kernel over LDS tiles: one workgroup of TS lanes per tile of TS cells.  The
 // tile's distinct neighbours (and the far stand-in after them) are copied to
 // LDS once, as two arrays of 16-byte halves (x, y and z, size: consecutive
 // records start in different bank groups, which halves bank conflicts
@@ -324,12 +324,12 @@ __device__ __forceinline__ double stage_size(double size) {
 // 16-bit list 8 slots per load, every pair's record coming from LDS.  A tile
 // with more distinct neighbours than the LDS holds takes the 32-bit lists and
 // global records instead.
-template <bool MIXED, int U>
-__global__ __launch_bounds__(TM_TILE) void k_sph_tile(
+template <bool MIXED, int U, int TS>
+__global__ __launch_bounds__(TS) void k_sph_tile(
     const double4 *__restrict__ pos, const int32_t *__restrict__ ctype_s, const TypeDev *__restrict__ types,
     int64_t n, int64_t cap, const uint32_t *__restrict__ nbr, int64_t nbr_elems, const uint32_t *__restrict__ count,
     const uint32_t *__restrict__ tile_idx, const uint32_t *__restrict__ tile_cnt,
-    const uint8_t *__restrict__ tile_cell, const uint16_t *__restrict__ nbr16, int64_t nbr16_elems,
+    const uint16_t *__restrict__ tile_cell, const uint16_t *__restrict__ nbr16, int64_t nbr16_elems,
     uint32_t lds_cap, double *__restrict__ dens, double *__restrict__ fx, double *__restrict__ fy,
     double *__restrict__ fz) {
   extern __shared__ double2 lds[];
@@ -342,24 +342,25 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
   // of an over-full tile are never used), so staging waits on one load
   // round trip for the indices instead of two.
   const uint32_t *ti = tile_idx + g * TM_TILE_MAX;
-  constexpr int SPEC = 3;  // 3 TM_TILE = 768 <= TM_TILE_MAX indices ahead
+  constexpr int SPEC = 3;  // 3 TS <= TM_TILE_MAX indices ahead
+  static_assert(SPEC * TS <= TM_TILE_MAX, "staging indices read ahead");
   uint32_t si[SPEC];
 #pragma unroll
-  for (int j = 0; j < SPEC; ++j) si[j] = ti[t + TM_TILE * j];
-  const uint32_t rank_cell = tile_cell[g * TM_TILE + t];
+  for (int j = 0; j < SPEC; ++j) si[j] = ti[t + TS * j];
+  const uint32_t rank_cell = tile_cell[g * TS + t];
   const uint32_t m = __builtin_amdgcn_readfirstlane(tile_cnt[g]);
   const bool staged = m < lds_cap;  // m records and the stand-in fit (TM_TILE_OVER never does)
   if (staged) {
 #pragma unroll
     for (int j = 0; j < SPEC; ++j) {
-      const uint32_t i = t + TM_TILE * j;
+      const uint32_t i = t + TS * j;
       if (i < m) {
         const double4 q = pos[si[j]];
         lxy[i] = make_double2(q.x, q.y);
         lzs[i] = make_double2(q.z, stage_size<MIXED>(q.w));
       }
     }
-    for (uint32_t i = t + TM_TILE * SPEC; i < m; i += TM_TILE) {
+    for (uint32_t i = t + TS * SPEC; i < m; i += TS) {
       const double4 q = pos[ti[i]];
       lxy[i] = make_double2(q.x, q.y);
       lzs[i] = make_double2(q.z, stage_size<MIXED>(q.w));
@@ -370,7 +371,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
     }
   }
   const uint32_t off = staged ? (uint32_t)rank_cell : t;
-  const int64_t cell = g * TM_TILE + off;
+  const int64_t cell = g * TS + off;
   const bool live = cell < n;
   const int64_t p = live ? cell : n - 1;
   const double4 P = pos[p];
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
   if (staged) {
     // every chunk below the wave's longest list is written (the padding
     // holds m, the stand-in's place)
-    const __amdgpu_buffer_rsrc_t r16 = list_rsrc(nbr16, tm_nbr16_at(g * TM_TILE, 0, cap), nbr16_elems);
+    const __amdgpu_buffer_rsrc_t r16 = list_rsrc(nbr16, tm_nbr16_at(g * TS, 0, cap), nbr16_elems);
     const uint32_t lane_off = (uint32_t)(tm_nbr16_at(t, 0, cap) * 2);
     const uint32_t chunks = (wmax + 7) >> 3;
     tm_u32x4 nx = {0u, 0u, 0u, 0u};
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
       if constexpr (MIXED) de += (double)sw;
     }
   } else {
-    const int64_t g0 = g * (TM_TILE / 64);
+    const int64_t g0 = g * (TS / 64);
     const __amdgpu_buffer_rsrc_t rl = list_rsrc(nbr, g0 * cap * 64, nbr_elems);
     const uint32_t lane_off = (uint32_t)((((p >> 6) - g0) * cap * 64 + (p & 63)) * 4);
     direct_sums<MIXED, U, 1, true, true>(pos, n, cap, rl, lane_off, 0, P, C64, K, cnt, wmax, de, f0, f1, f2);
@@ -431,7 +432,7 @@ __global__ __launch_bounds__(TM_TILE) void k_sph_tile(
   lxy[2 * off] = make_double2(de, f0);
   lxy[2 * off + 1] = make_double2(f1, f2);
   __syncthreads();
-  const int64_t c = g * TM_TILE + t;
+  const int64_t c = g * TS + t;
   if (c >= n) return;
   const double2 a = lxy[2 * t], b = lxy[2 * t + 1];
   dens[c] = a.x;
@@ -464,13 +465,25 @@ void launch_k(tm_ctx *c) {
       c->d_dens, c->d_fx, c->d_fy, c->d_fz);
 }
 
-template <bool MIXED, int U>
-void launch_tile(tm_ctx *c) {
+template <bool MIXED, int U, int TS>
+void launch_tile_ts(tm_ctx *c) {
   const int64_t n = c->n;
-  const int64_t nt = (n + TM_TILE - 1) / TM_TILE;
-  k_sph_tile<MIXED, U><<<(unsigned)nt, TM_TILE, 2 * sizeof(double2) * c->tile_lds, c->stream>>>(
+  const int64_t nt = (n + TS - 1) / TS;
+  k_sph_tile<MIXED, U, TS><<<(unsigned)nt, TS, 2 * sizeof(double2) * c->tile_lds, c->stream>>>(
       c->d_pos, c->d_ctype_s, c->d_types, n, c->nbr_alloc, c->d_nbr, c->nbr_elems, c->d_ncount, c->d_tile_idx,
       c->d_tile_cnt, c->d_tile_cell, c->d_nbr16, c->cap_nbr16, c->tile_lds, c->d_dens, c->d_fx, c->d_fy, c->d_fz);
+}
+
+// the tiles as built (their width, not the parameter's current value)
+template <bool MIXED, int U>
+void launch_tile(tm_ctx *c) {
+  // the pair kernel parks its TS cells' sums in the staging LDS
+  if (c->tile_lds < (uint32_t)c->tiles_ts) c->tile_lds = TM_TILE_MAX + 1;
+  switch (c->tiles_ts) {
+    case 128: launch_tile_ts<MIXED, U, 128>(c); break;
+    case 512: launch_tile_ts<MIXED, U, 512>(c); break;
+    default: launch_tile_ts<MIXED, U, 256>(c); break;
+  }
 }
 
 template <bool MIXED, int L>
@@ -494,7 +507,7 @@ void launch_sph(tm_ctx *c) {
     else launch_k<MIXED, 4, 1, false, false>(c);
     return;
   }
-  if (c->sph_tiles && c->tile_lds > 0) {
+  if (c->sph_tiles && c->tile_lds > 0 && c->tiles_ts > 0) {
     switch (MIXED ? c->sph_unroll_mixed : c->sph_unroll_fp64) {
       case 1:
       case 2: launch_tile<MIXED, 2>(c); break;

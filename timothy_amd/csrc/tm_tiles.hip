@@ -1,6 +1,7 @@
 // LDS tiles for the pair kernels, built from the neighbour lists.
 //
-// Cells are taken in tiles of TM_TILE consecutive cells of the Morton order.
+// Cells are taken in tiles of tile_width (128, 256 or 512; TM_TILE by
+// default) consecutive cells of the Morton order.
 // A tile's neighbours overlap heavily (in the 2M-cell Gaussian colony about
 // 600 distinct neighbours serve 4,100 pairs per tile), so the pair kernel
 // stages the distinct ones in LDS once and reads every pair from there
@@ -24,7 +25,6 @@
 
 namespace {
 
-constexpr int TS = TM_TILE;
 // Open-addressing set of the distinct neighbours outside a tile's bitmap
 // window.  A new key reserves a place before it is inserted, and a
 // reservation is granted only while fewer than RESERVE places are held.
@@ -34,17 +34,25 @@ constexpr int TS = TM_TILE;
 // Hence: the set never holds more than RESERVE keys, so at least
 // HASH - RESERVE slots stay empty and every probe ends; and a refused
 // reservation means at least RESERVE - (TS - 1) = TM_TILE_MAX + 1 keys are
-// in the set, i.e. the tile is truly over-full.
-constexpr int HASH = 2048;
-constexpr uint32_t RESERVE = TM_TILE_MAX + TS;
+// in the set, i.e. the tile is truly over-full.  TS is the tile width (the
+// workgroup size): 128, 256 or 512 cells.
+template <int TS>
+struct TileCfg {
+  static constexpr int HASH = TS <= 256 ? 2048 : 4096;
+  static constexpr int HBITS = TS <= 256 ? 11 : 12;
+  static constexpr uint32_t RESERVE = TM_TILE_MAX + TS;
+  static_assert(HASH == 1 << HBITS && HASH > (int)RESERVE, "hash slots");
+};
 constexpr int SORTN = 2048;  // power of two >= TM_TILE_MAX
 constexpr int WB = 16384;    // cells around a tile's middle whose order a bitmap gives
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t hslot(uint32_t q) { return (q * 2654435761u) >> 21; }  // 11 bits
-static_assert(HASH == 2048 && HASH > (int)RESERVE, "hash slots");
+template <int HBITS>
+__device__ __forceinline__ uint32_t hslot(uint32_t q) {
+  return (q * 2654435761u) >> (32 - HBITS);
+}
 
-// Ascending bitonic sort of list[0, m) (m <= 256 E) by the 256 threads of
+// Ascending bitonic sort of list[0, m) (m <= TS E) by the TS threads of
 // the workgroup, each holding E consecutive elements (padding EMPTY sorts
 // last).  A compare-exchange partner at distance j lies in the same thread
 // (j < E), in another lane of the same wave (j < 64 E: a shuffle), or in
@@ -56,9 +64,10 @@ __device__ __forceinline__ uint32_t bitonic_pick(uint32_t x, uint32_t y, uint32_
   return lower == asc ? min(x, y) : max(x, y);
 }
 
-template <int E>
+template <int TS, int E>
 __device__ void tile_sort(uint32_t *list, uint32_t m, uint32_t t) {
-  constexpr uint32_t N = (uint32_t)TM_TILE * E;
+  constexpr uint32_t N = (uint32_t)TS * E;
+  static_assert(N <= SORTN, "sort buffer");
   uint32_t a[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -102,7 +111,8 @@ __device__ void tile_sort(uint32_t *list, uint32_t m, uint32_t t) {
   __syncthreads();
 }
 
-// exclusive prefix sum over the 256 threads of the workgroup
+// exclusive prefix sum over the threads of the workgroup (wsum: one word
+// per wave)
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *wsum) {
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t inc = v;
@@ -120,16 +130,21 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *wsum) {
 }
 
 // win: WB, or 0 to send every key through the hash set (same outputs)
+template <int TS>
 __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t win, const uint32_t *__restrict__ nbr,
                                               const uint32_t *__restrict__ count, uint32_t *__restrict__ tile_idx,
-                                              uint32_t *__restrict__ tile_cnt, uint8_t *__restrict__ tile_cell,
+                                              uint32_t *__restrict__ tile_cnt, uint16_t *__restrict__ tile_cell,
                                               uint16_t *__restrict__ nbr16, unsigned int *__restrict__ flags) {
+  constexpr int HASH = TileCfg<TS>::HASH, HBITS = TileCfg<TS>::HBITS;
+  constexpr uint32_t RESERVE = TileCfg<TS>::RESERVE;
+  constexpr int WPT = WB / 32 / TS;  // bitmap words per thread
+  static_assert(WPT * TS * 32 == WB, "bitmap words");
   __shared__ uint32_t keys[HASH];
   __shared__ uint16_t vals[HASH];
   __shared__ uint32_t list[SORTN];
   __shared__ uint32_t bins[TS];
   __shared__ uint32_t rcnt[TS];
-  __shared__ uint8_t order[TS];
+  __shared__ uint16_t order[TS];
   __shared__ uint32_t wsum[TS / 64];
   __shared__ uint32_t bm[WB / 32], pre[WB / 32];
   __shared__ uint32_t distinct, nwin, nbelow, hfull;
@@ -153,7 +168,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   bins[t] = start;
   __syncthreads();
   const uint32_t rank = atomicAdd(&bins[b], 1u);
-  order[rank] = (uint8_t)t;
+  order[rank] = (uint16_t)t;
   rcnt[rank] = cnt;
 
   // Distinct neighbours of the tile.  Most lie within WB / 2 cells of its
@@ -180,7 +195,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
       }
       // look it up first (most occurrences are repeats); slots are never
       // emptied, so at most TM_TILE_MAX of the HASH slots read are taken
-      uint32_t h = hslot(q);
+      uint32_t h = hslot<HBITS>(q);
       uint32_t k = keys[h];
       while (k != q && k != EMPTY) {
         h = (h + 1) & (HASH - 1);
@@ -206,14 +221,23 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
     }
   }
   __syncthreads();
-  // the window's count and prefix
-  static_assert(WB / 32 == 2 * TS, "two bitmap words per thread");
-  const uint32_t b0 = bm[2 * t], b1 = bm[2 * t + 1];
-  const uint32_t c0 = __popc(b0), c1 = __popc(b1);
-  const uint32_t pw = block_scan(c0 + c1, wsum);
-  pre[2 * t] = pw;
-  pre[2 * t + 1] = pw + c0;
-  if (t == TS - 1) nwin = pw + c0 + c1;
+  // the window's count and prefix (thread t holds words WPT t .. WPT t + WPT - 1)
+  uint32_t bw[WPT], cw = 0;
+#pragma unroll
+  for (int e = 0; e < WPT; ++e) {
+    bw[e] = bm[WPT * t + e];
+    cw += __popc(bw[e]);
+  }
+  const uint32_t pw = block_scan(cw, wsum);
+  {
+    uint32_t run = pw;
+#pragma unroll
+    for (int e = 0; e < WPT; ++e) {
+      pre[WPT * t + e] = run;
+      run += __popc(bw[e]);
+    }
+  }
+  if (t == TS - 1) nwin = pw + cw;
   __syncthreads();
   const uint32_t mo = distinct, mw = nwin, m = mo + mw;
   if (m > TM_TILE_MAX || hfull) {
@@ -240,28 +264,30 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   }
   __syncthreads();
   const uint32_t mb = nbelow;
-  static_assert(TS == 256 && 8 * TS >= TM_TILE_MAX && SORTN >= 8 * TS, "tile_sort sizes");
+  // mo <= TM_TILE_MAX <= SORTN: the largest sort below has TS E = SORTN
+  static_assert(SORTN >= TM_TILE_MAX && SORTN % TS == 0, "tile_sort sizes");
   if (mo == 0) {
     // all in the window: nothing to sort
-  } else if (mo <= TS) tile_sort<1>(list, mo, t);
-  else if (mo <= 2 * TS) tile_sort<2>(list, mo, t);
-  else if (mo <= 4 * TS) tile_sort<4>(list, mo, t);
-  else tile_sort<8>(list, mo, t);
+  } else if (mo <= TS) tile_sort<TS, 1>(list, mo, t);
+  else if (mo <= 2 * TS) tile_sort<TS, 2>(list, mo, t);
+  else if (TS * 4 <= SORTN && mo <= 4 * TS) tile_sort<TS, (TS * 4 <= SORTN ? 4 : 1)>(list, mo, t);
+  else if (TS * 8 <= SORTN && mo <= 8 * TS) tile_sort<TS, (TS * 8 <= SORTN ? 8 : 1)>(list, mo, t);
+  else tile_sort<TS, SORTN / TS>(list, mo, t);
   // positions: the sorted ones ...
   for (uint32_t i = t; i < mo; i += TS) {
     const uint32_t q = list[i];
     const uint32_t pos = i < mb ? i : i + mw;
-    uint32_t h = hslot(q);
+    uint32_t h = hslot<HBITS>(q);
     while (keys[h] != q) h = (h + 1) & (HASH - 1);
     vals[h] = (uint16_t)pos;
     tile_idx[g * TM_TILE_MAX + pos] = q;
   }
-  // ... and the window's, in order (thread t holds words 2t and 2t + 1)
+  // ... and the window's, in order
   {
     uint32_t pos = mb + pw;
-    for (int e = 0; e < 2; ++e) {
-      uint32_t bits = e ? b1 : b0;
-      const int64_t q0 = wlo + 32 * (2 * (int64_t)t + e);
+    for (int e = 0; e < WPT; ++e) {
+      uint32_t bits = bw[e];
+      const int64_t q0 = wlo + 32 * (WPT * (int64_t)t + e);
       while (bits) {
         const uint32_t bt = (uint32_t)__ffs(bits) - 1u;
         bits &= bits - 1u;
@@ -283,7 +309,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   uint32_t pad = rc;  // longest list of this wave of ranks
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) pad = max(pad, (uint32_t)__shfl_xor((int)pad, o, 64));
-  tile_cell[g * TS + t] = (uint8_t)off;
+  tile_cell[g * TS + t] = (uint16_t)off;
   const int64_t r = g * TS + t;
   const uint32_t pad8 = (pad + 7) & ~7u;  // <= cap, a multiple of 8
   // one 16-byte chunk of 8 slots per round: its 8 list loads overlap
@@ -302,7 +328,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
           const uint32_t wd = (uint32_t)(rr >> 5), bt = (uint32_t)(rr & 31);
           v[j] = mb + pre[wd] + __popc(bm[wd] & ((1u << bt) - 1u));
         } else {
-          uint32_t h = hslot(q);
+          uint32_t h = hslot<HBITS>(q);
           while (keys[h] != q) h = (h + 1) & (HASH - 1);
           v[j] = vals[h];
         }
@@ -320,24 +346,36 @@ __global__ void k_zero_tile_flags(unsigned int *flags) {
   flags[14] = 0;
 }
 
+template <int TS>
+void launch_tiles(tm_ctx *c, int64_t nt) {
+  k_tiles<TS><<<(unsigned)nt, TS, 0, c->stream>>>(c->n, c->nbr_alloc, c->tile_window ? (uint32_t)WB : 0u, c->d_nbr,
+                                                   c->d_ncount, c->d_tile_idx, c->d_tile_cnt, c->d_tile_cell,
+                                                   c->d_nbr16, c->d_flags);
+}
+
 }  // namespace
 
 tm_status tm_run_tiles(tm_ctx *c, bool check) {
   const int64_t n = c->n;
   if (!c->sph_tiles || n == 0) {
     c->tile_lds = 0;
+    c->tiles_ts = 0;
     return TM_OK;
   }
-  const int64_t nt = (n + TM_TILE - 1) / TM_TILE;
+  const int TS = c->tile_width;
+  const int64_t nt = (n + TS - 1) / TS;
   TM_TRY(tm_ensure(c, &c->d_tile_idx, c->cap_tile_idx, nt * TM_TILE_MAX, "tile indices"));
   TM_TRY(tm_ensure(c, &c->d_tile_cnt, c->cap_tile_cnt, nt, "tile sizes"));
-  TM_TRY(tm_ensure(c, &c->d_tile_cell, c->cap_tile_cell, nt * TM_TILE, "tile ranks"));
-  TM_TRY(tm_ensure(c, &c->d_nbr16, c->cap_nbr16, tm_slot_elems(nt * TM_TILE, c->nbr_alloc), "tile lists"));
+  TM_TRY(tm_ensure(c, &c->d_tile_cell, c->cap_tile_cell, nt * TS, "tile ranks"));
+  TM_TRY(tm_ensure(c, &c->d_nbr16, c->cap_nbr16, tm_slot_elems(nt * TS, c->nbr_alloc), "tile lists"));
   k_zero_tile_flags<<<1, 1, 0, c->stream>>>(c->d_flags);
-  k_tiles<<<(unsigned)nt, TM_TILE, 0, c->stream>>>(n, c->nbr_alloc, c->tile_window ? (uint32_t)WB : 0u, c->d_nbr,
-                                                    c->d_ncount, c->d_tile_idx, c->d_tile_cnt, c->d_tile_cell,
-                                                    c->d_nbr16, c->d_flags);
+  switch (TS) {
+    case 128: launch_tiles<128>(c, nt); break;
+    case 512: launch_tiles<512>(c, nt); break;
+    default: launch_tiles<256>(c, nt); break;
+  }
   TM_HIP(c, hipGetLastError());
+  c->tiles_ts = TS;
   if (check) {
     TM_HIP(c, hipMemcpyAsync(c->h_flags + 4, c->d_flags + 4, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost,
                              c->stream));
@@ -346,16 +384,19 @@ tm_status tm_run_tiles(tm_ctx *c, bool check) {
     TM_HIP(c, hipStreamSynchronize(c->stream));
     c->tiles_over = c->h_flags[14];
     // records staged per tile: the largest tile plus the far stand-in, rounded
-    // (at least TM_TILE: the pair kernel also parks a tile's sums there) ...
-    int64_t lds = std::max<int64_t>(TM_TILE,
-                                    std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll));
+    // (at least TS: the pair kernel also parks a tile's sums there) ...
+    int64_t lds = std::max<int64_t>(TS, std::min<int64_t>(TM_TILE_MAX + 1, ((int64_t)c->h_flags[4] + 1 + 63) & ~63ll));
     // ... but no more than four workgroups' worth per CU when at most one tile
     // in twenty is larger: those take the pair kernel's global-record path and
     // every other tile runs at the higher occupancy
     if (lds > TM_TILE_LDS4 && c->tile_cap4 && 20 * (int64_t)c->h_flags[5] <= nt) lds = TM_TILE_LDS4;
     c->tile_lds = (uint32_t)lds;
-  } else if (c->tile_lds == 0) {
+    c->tile_lds_ts = TS;
+  } else if (c->tile_lds == 0 || c->tile_lds_ts != TS) {
+    // unsized, or sized for another width (the sums of TS cells must fit):
+    // stage the most a tile can hold
     c->tile_lds = TM_TILE_MAX + 1;
+    c->tile_lds_ts = TS;
   }
   return TM_OK;
 }
