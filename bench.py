@@ -127,6 +127,37 @@ def cpu_walks(col, threads, reps=3, deposit=False):
     return nthr, t_build, tp, tg, t_dep
 
 
+def compat_step(col, reps=3):
+    """one step through the reference's C API (libtimothy_compat.so) from a
+    host caller, as a reference build linking it would run: wall time per
+    call, PCIe copies included (the compat step timer, a child process)"""
+    import subprocess
+    import tempfile
+    from timothy_amd import build as B
+    if not os.path.exists(B.STEP_TIMER):
+        return None
+    grid = col["grid"]
+    types = col["types"]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cells.bin")
+        with open(path, "wb") as f:
+            np.array([col["n"], len(types), grid["G"]], np.int64).tofile(f)
+            np.array([[t["size"], t["h"], t["h2"], t["h3"], t["h4"]] for t in types], np.float64).tofile(f)
+            np.array(list(grid["lower"]) + [float(grid["resolution"])], np.float64).tofile(f)
+            np.stack([col["x"], col["y"], col["z"], col["size"]], 1).astype(np.float64).tofile(f)
+            np.asarray(col["young"], np.float32).tofile(f)
+            np.asarray(col["ctype"], np.int32).tofile(f)
+            np.asarray(col["phase"], np.int32).tofile(f)
+        r = subprocess.run([B.STEP_TIMER, path, str(reps)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        log(f"compat step timer failed ({r.returncode}): {r.stderr[-500:]}")
+        return None
+    out = json.loads(r.stdout)
+    out["note"] = ("octbuild .. fieldgradient in src/step.c order through libtimothy_compat.so from host "
+                   "celldata_t arrays, one 256^3 FP64 field, FP64 sums; median of the reps after a warm-up")
+    return out
+
+
 def gpu_pairs(tm, col):
     with tm.Context(0) as c:
         c.upload_colony(col)
@@ -330,6 +361,7 @@ def run_single(a, tm, gen):
     loop_ms = (time.perf_counter() - t0) * 1e3 / nloop
     ctx.graph_destroy()
 
+    compat = compat_step(col)
     cpu = cpu1 = None
     if not a.no_cpu:
         nthr, t_build, tp, tg, t_dep = cpu_walks(col, a.cpu_threads, deposit=True)
@@ -376,6 +408,7 @@ def run_single(a, tm, gen):
                          "sph_mixed": sph_mixed, "cic_deposit_256^3": cic_ms},
         "like_for_like": lfl,
         "mechanics_loop_ms_per_step": loop_ms,
+        "compat_step_host_arrays": compat,
         "environment_solve_256^3": env,
         "copy_kernel_gbs": copy_gbs,
         "roofline": roofline(B, mixed["ms"], traffic_from_profiles()),

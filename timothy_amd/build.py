@@ -68,12 +68,24 @@ def build(verbose: bool = False) -> None:
           os.path.join(CSRC, "compat", "tm_compat.c"), "-o", tmp,
           "-L" + HERE, "-ltimothy_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
     os.replace(tmp, COMPAT)
+    build_step_timer()
     build_step_harness()
     if verbose:
         print("built", LIB, COMPAT, HARNESS)
 
 
 HARNESS = os.path.join(ROOT, "tests", "compat", "_build", "step_harness")
+STEP_TIMER = os.path.join(OUT, "step_timer")
+
+
+def build_step_timer() -> None:
+    """the program bench.py times the compat step with (a reference-style
+    host caller of libtimothy_compat.so)"""
+    tmp = STEP_TIMER + ".tmp"
+    _run(["gcc", "-std=gnu11", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "compat", "step_timer.c"), "-o", tmp, "-L" + HERE, "-ltimothy_compat", "-ltimothy_hip",
+          "-Wl,-rpath,$ORIGIN/..", "-lm"])
+    os.replace(tmp, STEP_TIMER)
 
 
 ENV_HARNESS = os.path.join(ROOT, "tests", "compat", "_build", "env_harness")
