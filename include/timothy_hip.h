@@ -121,7 +121,11 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * by bitmap and hashes only the rest; 0 = hashes all of them; results are
  * identical; default 1), "tile_width" cells per LDS tile (128, 256, 512;
  * default 256; takes effect at the next tm_neighbors; results are
- * identical); a change of "nbr_cap" or "tile_width" retires a captured graph;
+ * identical), "walk_coop" (1 = each wave of 64 cells walks the tree once for
+ * the union of its query boxes, 64 slots per load, and each lane picks its
+ * own candidates from the union's; waves spanning two colonies or whose
+ * union box exceeds "walk_coop_ratio" (default 64) times their mean box
+ * volume walk per lane; results are identical; default 0); a change of "nbr_cap" or "tile_width" retires a captured graph;
  * read-only "nbr_alloc" (slots per cell of the
  * current lists), "tile_lds" (records staged per tile, 0 without tiles) and
  * "tiles_over" (tiles with more than 1536 distinct neighbours, which take
@@ -328,10 +332,10 @@ tm_status tm_graph_destroy(tm_ctx *ctx);
 
 /* -------- autotuner -------- */
 /* Times candidate shapes for the current colonies and keeps the fastest: the
- * walk (nbr_block x walk_window, then walk_f32), then with LDS tiles (3-D)
- * tile_width (128, 256, 512) x sph_unroll timed as tile build + pair kernel,
- * else sph_block x sph_unroll (x sph_lanes, mixed).  No candidate changes a
- * result.  Leaves the lists and tiles built with the chosen shapes.  Report
+ * walk (nbr_block x walk_window, then walk_coop, then walk_f32), then with
+ * LDS tiles (3-D) tile_width (128, 256, 512) x sph_unroll timed as tile
+ * build + pair kernel, else sph_block x sph_unroll (x sph_lanes, mixed).
+ * No candidate changes a result.  Leaves the lists and tiles built with the chosen shapes.  Report
  * into buf (may be NULL): one line "trial <stage> key=value ... ms=<t>" per
  * candidate, then "chosen key=value ...".  Replaces the reference's only
  * knob, the OpenMP schedule of the walks (src/potential.c:90-93). */

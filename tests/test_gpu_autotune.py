@@ -104,7 +104,8 @@ def test_autotune_report_and_results(tm, ctx, case):
         assert all(t["ms"] > 0 for t in rep["trials"])
         ch = rep["chosen"]
         assert ch["precision"] == ("fp64" if prec == tm.FP64 else "mixed")
-        for k in ("nbr_block", "walk_window", "walk_f32", "tile_width"):
+        assert {t.get("walk_coop") for t in rep["trials"] if "walk_coop" in t} == {0, 1}
+        for k in ("nbr_block", "walk_window", "walk_coop", "walk_f32", "tile_width"):
             assert ctx.get_param(k) == ch[k], k
         best = min((t for t in rep["trials"] if t["stage"] == "tiles_sph"), key=lambda t: t["ms"])
         assert (best["tile_width"], best["unroll"]) == (ch["tile_width"], ch["unroll"])

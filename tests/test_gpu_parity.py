@@ -640,3 +640,60 @@ def test_full_size_properties(tm, ctx, precision):
     prod = tm.DeviceArray(ctx, grid["G"] ** 3, np.float64)
     ctx.cic_deposit(grid, prod, 0)
     assert abs(prod.get().sum() - col["n"]) <= 1e-9 * col["n"]
+
+
+@pytest.mark.parametrize("name,make", CASES)
+def test_cooperative_walk_exact(tm, oracle_mod, ctx, name, make):
+    """one union walk per wave (walk_coop) gives the per-lane walk's lists,
+    qstar, mindist and v, and bit-exact FP64 sums: with every wave
+    cooperative (ratio 1e6), with the default split, and with almost every
+    wave on the per-lane path (ratio 1); FP32 screening on and off"""
+    O = oracle_mod
+    col = make()
+    ctx.upload_colony(col)
+    ctx.build()
+    tree = O.Octree(O.Colony.from_dict(col), col["types"])
+    ref = oracle_results(tree)
+    ctx.set_param("walk_coop", 0)
+    ctx.neighbors()
+    lists, q0 = ctx.neighbor_lists(), ctx.qstar()
+    try:
+        ctx.set_param("walk_coop", 1)
+        for f32 in (1, 0):
+            ctx.set_param("walk_f32", f32)
+            for ratio in (1000000, 64, 1):
+                say(f"coop walk {name}: f32 {f32} ratio {ratio}")
+                ctx.set_param("walk_coop_ratio", ratio)
+                ctx.neighbors()
+                got = ctx.neighbor_lists()
+                assert np.array_equal(got[0], lists[0]) and np.array_equal(got[1], lists[1])
+                assert np.array_equal(ctx.qstar(), q0)
+                ctx.sph(tm.FP64)
+                r = ctx.results()
+                for k in ("density", "v", "mindist", "force"):
+                    assert np.array_equal(r[k], ref[k]), k
+    finally:
+        ctx.set_param("walk_coop", 0)
+        ctx.set_param("walk_f32", 1)
+        ctx.set_param("walk_coop_ratio", 64)
+
+
+def test_cooperative_walk_batched(tm, oracle_mod, ctx):
+    """batched colonies under the cooperative walk: waves that span two
+    colonies walk per lane; every colony bit-exact against its own oracle run"""
+    O = oracle_mod
+    cols = gen.colonies_config3(ncol=12, scale=0.05)
+    off = ctx.upload_colonies(cols)
+    try:
+        ctx.set_param("walk_coop", 1)
+        ctx.set_param("walk_coop_ratio", 1000000)
+        ctx.step(tm.FP64)
+        res = ctx.results()
+        for i, c in enumerate(cols):
+            sl = slice(off[i], off[i + 1])
+            ref = oracle_results(O.Octree(O.Colony.from_dict(c), c["types"]))
+            for k in ("density", "v", "mindist", "force"):
+                assert np.array_equal(res[k][sl], ref[k]), (i, k)
+    finally:
+        ctx.set_param("walk_coop", 0)
+        ctx.set_param("walk_coop_ratio", 64)

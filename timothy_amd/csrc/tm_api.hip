@@ -436,6 +436,11 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     c->walk_f32 = v ? 1 : 0;
   } else if (!strcmp(key, "tile_window")) {
     c->tile_window = v ? 1 : 0;
+  } else if (!strcmp(key, "walk_coop")) {
+    c->walk_coop = v ? 1 : 0;
+  } else if (!strcmp(key, "walk_coop_ratio")) {
+    if (v < 1 || v > 1000000) return tm_fail(c, TM_ERR_INVALID, "walk_coop_ratio in [1, 1e6]");
+    c->walk_coop_ratio = (int)v;
   } else if (!strcmp(key, "tile_width")) {
     if (v != 128 && v != 256 && v != 512) return tm_fail(c, TM_ERR_INVALID, "tile_width in {128,256,512}");
     if (v != c->tile_width) tm_graph_destroy(c);
@@ -465,6 +470,8 @@ tm_status tm_get_param(tm_ctx *c, const char *key, int64_t *v) {
   else if (!strcmp(key, "walk_f32")) *v = c->walk_f32;
   else if (!strcmp(key, "tile_window")) *v = c->tile_window;
   else if (!strcmp(key, "tile_width")) *v = c->tile_width;
+  else if (!strcmp(key, "walk_coop")) *v = c->walk_coop;
+  else if (!strcmp(key, "walk_coop_ratio")) *v = c->walk_coop_ratio;
   else if (!strcmp(key, "tile_lds")) *v = c->tile_lds;
   else if (!strcmp(key, "tiles_over")) *v = c->tiles_over;
   else if (!strcmp(key, "nbr_alloc")) *v = c->nbr_alloc;
@@ -926,6 +933,18 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   }
   c->nbr_block = best_nb;
   c->walk_window = best_win;
+  // that shape against one union walk per wave (same results)
+  {
+    double tl = 0, tc = 0;
+    TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 3, &tl));
+    const int c0 = c->walk_coop;
+    c->walk_coop = 1;
+    TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 3, &tc));
+    snprintf(line, sizeof(line), "trial neighbors walk_coop=0 ms=%.4f\ntrial neighbors walk_coop=1 walk_coop_ratio=%d ms=%.4f\n",
+             tl, c->walk_coop_ratio, tc);
+    rep += line;
+    c->walk_coop = tc < tl || (c0 && tc <= tl) ? 1 : 0;
+  }
   // that shape with the FP32 candidate screening on and off (same results)
   {
     const int f0 = c->walk_f32;
@@ -998,10 +1017,10 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
     if (p == TM_MIXED) c->sph_lanes = best_l;
   }
   snprintf(line, sizeof(line),
-           "chosen precision=%s nbr_block=%d walk_window=%d walk_f32=%d sph_tiles=%d tile_width=%d sph_block=%d "
-           "unroll=%d lanes=%d\n",
-           p == TM_MIXED ? "mixed" : "fp64", c->nbr_block, c->walk_window, c->walk_f32, tiles ? 1 : 0, c->tile_width,
-           c->sph_block, unroll, c->sph_lanes);
+           "chosen precision=%s nbr_block=%d walk_window=%d walk_coop=%d walk_f32=%d sph_tiles=%d tile_width=%d "
+           "sph_block=%d unroll=%d lanes=%d\n",
+           p == TM_MIXED ? "mixed" : "fp64", c->nbr_block, c->walk_window, c->walk_coop, c->walk_f32, tiles ? 1 : 0,
+           c->tile_width, c->sph_block, unroll, c->sph_lanes);
   rep += line;
   if (buf && buflen) {
     strncpy(buf, rep.c_str(), buflen - 1);

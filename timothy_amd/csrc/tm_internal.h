@@ -156,6 +156,8 @@ struct tm_ctx {
   uint16_t *d_tile_cell = nullptr;  // per tile: cell offset of each rank (longest list first)
   uint16_t *d_nbr16 = nullptr;      // slot k of rank r of tile g at tm_nbr16_at(g*TS + r, k, nbr_alloc)
   int tile_width = TM_TILE;         // cells per tile for the next tile build: 128, 256 or 512
+  int walk_coop = 0;                // 1: one union walk per wave (see k_neighbors)
+  int walk_coop_ratio = 64;         // union box volume / mean cell box volume above which a wave walks per lane
   int tiles_ts = 0;                 // cells per tile of the tiles as built (0: none)
   int tile_lds_ts = 0;              // the tile width tile_lds was sized for
   int64_t cap_tile_idx = 0, cap_tile_cnt = 0, cap_tile_cell = 0, cap_nbr16 = 0;

@@ -86,7 +86,25 @@ __device__ __forceinline__ double dist2_fp64(const double4 &P, const double4 &Q)
 // that place (the newcomer, or the one it displaces) is read in FP64 at once
 // when it is still within reach of the smallest (rare), and the smallest is
 // read in FP64 at the end.
-template <int BS, int WIN, bool F32>
+// COOP: each wave (BS = 64) first walks the tree once for the union of its
+// cells' query boxes, reading the pre-ordered slots 64 at a time (one
+// coalesced load for all lanes, where the per-lane walk gathers one record
+// per lane per node).  A slot is visited when no missed internal node before
+// it in the window skips it (a prefix maximum of the missed nodes' escape
+// slots); the walk continues after the window or at the largest escape.
+// The leaves it finds that overlap the union box are, in visit order, the
+// wave's union candidates: every leaf whose octant overlaps a cell's own box
+// overlaps the union box, and its ancestors contain its octant, so a cell's
+// candidates are exactly the union candidates that overlap its own box, in
+// the same order.  They are staged in LDS (CB at a time with their FP32 or
+// FP64 records) and each lane takes its own from there -- its box test over
+// 64 staged candidates at a time as a bit mask, then the set bits in order.
+// A wave whose cells span two colonies, or whose union box is more than
+// coop_ratio times its cells' mean box volume (a Morton jump: the union would
+// hold far more candidates than any cell), walks per lane as without COOP.
+constexpr int CB = 128;  // union candidates staged per wave
+
+template <int BS, int WIN, bool F32, bool COOP>
 __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ pos, const float4 *__restrict__ posf,
                                                   const uint4 *__restrict__ code,
                                                   const int32_t *__restrict__ ctype_s,
@@ -98,7 +116,8 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
                                                   uint32_t *__restrict__ nbr,
                                                   uint32_t *__restrict__ count, double *__restrict__ mindist,
                                                   int32_t *__restrict__ qstar, double *__restrict__ vout,
-                                                  unsigned int *__restrict__ flags) {
+                                                  unsigned int *__restrict__ flags, float coop_ratio) {
+  static_assert(!COOP || (BS == 64 && WIN == 0), "a cooperative walk takes one wave per workgroup, no window");
   const int64_t cell = tm_xcd_block() * BS + threadIdx.x;
   const bool live = cell < n;
   const int64_t p = live ? cell : n - 1;  // idle lanes only join the padding step below
@@ -217,6 +236,123 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   // issued, so waiting for them does not also wait for the store (vector
   // memory counters retire in order).  Candidates are still taken in visit
   // order.
+  bool coop = false;
+  if constexpr (COOP) {
+    // the wave's union box, over its live cells of one colony
+    const uint32_t lane = threadIdx.x;
+    uint32_t u[6] = {live ? mn0 : ~0u, live ? mn1 : ~0u, live ? mn2 : ~0u, live ? mx0 : 0u, live ? mx1 : 0u,
+                     live ? mx2 : 0u};
+    uint32_t cmin = live ? cid : ~0u, cmax = live ? cid : 0u;
+    float vl = live ? (float)(mx0 - mn0) * (float)(mx1 - mn1) * (float)(mx2 - mn2) : 0.f;
+    float nl = live ? 1.f : 0.f;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) u[a] = min(u[a], (uint32_t)__shfl_xor((int)u[a], o, 64));
+#pragma unroll
+      for (int a = 3; a < 6; ++a) u[a] = max(u[a], (uint32_t)__shfl_xor((int)u[a], o, 64));
+      cmin = min(cmin, (uint32_t)__shfl_xor((int)cmin, o, 64));
+      cmax = max(cmax, (uint32_t)__shfl_xor((int)cmax, o, 64));
+      vl += __shfl_xor(vl, o, 64);
+      nl += __shfl_xor(nl, o, 64);
+    }
+    const float vu = (float)(u[3] - u[0]) * (float)(u[4] - u[1]) * (float)(u[5] - u[2]);
+    coop = nl > 0.f && cmin == cmax && vu <= coop_ratio * (vl / nl);
+    if (coop) {
+      __shared__ uint4 crec[CB];  // lo codes x, y, z and the cell
+      __shared__ uint4 csz[CB];   // octant sizes x, y, z
+      __shared__ QRec cq[CB];     // the cell's record
+      const uint32_t uex0 = u[3] - 1u, uex1 = u[4] - 1u, uex2 = u[5] - 1u;
+      const uint32_t udm0 = u[3] - u[0] - 1u, udm1 = u[4] - u[1] - 1u, udm2 = u[5] - u[2] - 1u;
+      const uint32_t wbeg = (uint32_t)(2 * col_off[cmin]), wend = (uint32_t)(2 * col_off[cmin + 1] - 1);
+      uint32_t nb = 0;
+      // each lane's candidates among the staged ones, in order
+      auto drain = [&]() {
+        for (uint32_t e = lane; e < nb; e += 64) {
+          if constexpr (F32) cq[e] = posf[crec[e].w];
+          else cq[e] = pos[crec[e].w];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t b0 = 0; b0 < nb; b0 += 64) {
+          const uint32_t m = min(64u, nb - b0);
+          uint64_t mask = 0;
+          for (uint32_t j = 0; j < m; ++j) {
+            const uint4 r = crec[b0 + j], z = csz[b0 + j];
+            const bool hit = (ex0 - r.x < dm0 + z.x) & (ex1 - r.y < dm1 + z.y) & (ex2 - r.z < dm2 + z.z);
+            mask |= (uint64_t)(hit && live && r.w != (uint32_t)p) << j;
+          }
+          while (mask) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(mask);
+            mask &= mask - 1;
+            const uint32_t q = crec[b0 + j].w;
+            const int64_t slot = visit(q, cq[b0 + j]);
+            if (slot >= 0) nbr[slot] = q;
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        nb = 0;
+      };
+      // the window after this one is loaded with it: taken when the walk
+      // goes on there (no escape past it)
+      uint4 nxt = wbeg + lane < wend ? wn[wbeg + lane] : make_uint4(0u, 0u, 0u, 0u);
+      for (uint32_t cur = wbeg; cur < wend;) {
+        const uint32_t sl = cur + lane;
+        const bool valid = sl < wend;
+        const uint4 a = nxt;
+        nxt = sl + 64u < wend ? wn[sl + 64u] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t L = (a.x >> 29) | (((a.y >> 29) & 3u) << 3);
+        const uint32_t lx = a.x & 0x1FFFFFFFu, ly = a.y & 0x1FFFFFFFu, lz = a.z & 0x1FFFFFFFu;
+        const uint32_t sx = 1u << L, sy = sx >> ((a.z >> 30) & 1u), sz = sx >> ((a.z >> 29) & 1u);
+        const bool hitu = (uex0 - lx < udm0 + sx) & (uex1 - ly < udm1 + sy) & (uex2 - lz < udm2 + sz);
+        const bool leaf = (a.y & 0x80000000u) != 0u;
+        const bool miss = valid && !hitu && !leaf;
+        // slots skipped by a missed node earlier in the window
+        uint32_t pm = miss ? a.w : 0u;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = (uint32_t)__shfl_up((int)pm, o, 64);
+          if (lane >= (uint32_t)o) pm = max(pm, y);
+        }
+        uint32_t before = (uint32_t)__shfl_up((int)pm, 1, 64);
+        if (lane == 0) before = 0u;
+        const uint32_t mall = (uint32_t)__shfl((int)pm, 63, 64);
+        const bool visited = valid && before <= sl;
+        // a malformed tree: a visited miss that does not move forward, or a
+        // leaf naming no cell
+        if (__ballot(visited && ((miss && a.w <= sl) || (leaf && a.w >= (uint32_t)n)))) {
+          if (lane == 0) atomicOr(flags, (unsigned)TM_FLAG_WALK);
+          break;
+        }
+        const bool cand = visited && leaf && hitu;
+        const uint64_t cm = __ballot(cand);
+        const uint32_t k = (uint32_t)__builtin_popcountll(cm);
+        if (k) {
+          if (nb + k > (uint32_t)CB) drain();
+          if (cand) {
+            const uint32_t e = nb + (uint32_t)__builtin_popcountll(cm & tm_lanemask_lt());
+            crec[e] = make_uint4(lx, ly, lz, a.w);
+            csz[e] = make_uint4(sx, sy, sz, 0u);
+          }
+          nb += k;
+        }
+        if (mall > cur + 64u) {
+          cur = mall;
+          nxt = cur + lane < wend ? wn[cur + lane] : make_uint4(0u, 0u, 0u, 0u);
+        } else {
+          cur += 64u;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (nb) drain();
+      node = end;  // the per-lane walk below has nothing left to do
+    }
+  }
   uint32_t pq = 0xFFFFFFFFu;  // the candidate whose record is in flight
   QRec Qp{};
   while (node < end) {
@@ -288,12 +424,12 @@ __global__ __launch_bounds__(BS) void k_neighbors(const double4 *__restrict__ po
   if (cnt > cap) atomicMax(flags + 1, cnt);
 }
 
-template <int BS, int WIN>
+template <int BS, int WIN, bool COOP = false>
 void launch_walk(tm_ctx *c) {
-  auto kern = c->walk_f32 ? k_neighbors<BS, WIN, true> : k_neighbors<BS, WIN, false>;
+  auto kern = c->walk_f32 ? k_neighbors<BS, WIN, true, COOP> : k_neighbors<BS, WIN, false, COOP>;
   kern<<<tm_blocks(c->n, BS), BS, 0, c->stream>>>(
       c->d_pos, c->d_posf, c->d_code, c->d_ctype_s, c->d_young_s, c->d_wn, c->d_col_off, c->d_aff, c->d_types, c->n,
-      c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_v, c->d_flags);
+      c->nbr_alloc, c->d_nbr, c->d_ncount, c->d_mind, c->d_qstar, c->d_v, c->d_flags, (float)c->walk_coop_ratio);
 }
 
 }  // namespace
@@ -320,7 +456,8 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     }
     c->nbr_alloc = c->nbr_cap;
     const int bs = c->nbr_block;
-    if (bs == 64) launch_walk<64, 0>(c);
+    if (c->walk_coop) launch_walk<64, 0, true>(c);
+    else if (bs == 64) launch_walk<64, 0>(c);
     else if (bs == 128) launch_walk<128, 0>(c);
     else if (c->walk_window == 2048 && 2 * n - 1 >= 2048) launch_walk<256, 2048>(c);
     else if (c->walk_window == 1024 && 2 * n - 1 >= 1024) launch_walk<256, 1024>(c);
