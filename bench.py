@@ -75,7 +75,13 @@ def autotune(tm, ctx, a):
     t0 = time.perf_counter()
     r = ctx.autotune(tm.MIXED)
     log(r["text"].splitlines()[-1])
-    return {"chosen": r["chosen"], "trials": len(r["trials"]), "seconds": round(time.perf_counter() - t0, 3)}
+    # the tile width x unroll trials are kept: the tuner minimises tile build
+    # + pair kernel (both run once per step), which can cost the pair kernel
+    # alone some time where a wider tile builds faster
+    ts = [{k: t[k] for k in ("tile_width", "unroll", "tiles_ms", "sph_ms")} for t in r["trials"]
+          if t["stage"] == "tiles_sph"]
+    return {"chosen": r["chosen"], "trials": len(r["trials"]), "tiles_sph_trials": ts,
+            "seconds": round(time.perf_counter() - t0, 3)}
 
 
 def traffic_from_profiles():

@@ -331,7 +331,8 @@ tm_status tm_graph_launch(tm_ctx *ctx);
 tm_status tm_graph_destroy(tm_ctx *ctx);
 
 /* -------- autotuner -------- */
-/* Times candidate shapes for the current colonies and keeps the fastest: the
+/* Times candidate shapes for the current colonies and keeps the fastest (the
+ * current shape stays unless a candidate is more than 2% faster): the
  * walk (nbr_block x walk_window, then walk_coop, then walk_f32), then with
  * LDS tiles (3-D) tile_width (128, 256, 512) x sph_unroll timed as tile
  * build + pair kernel, else sph_block x sph_unroll (x sph_lanes, mixed).

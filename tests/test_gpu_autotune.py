@@ -107,8 +107,10 @@ def test_autotune_report_and_results(tm, ctx, case):
         assert {t.get("walk_coop") for t in rep["trials"] if "walk_coop" in t} == {0, 1}
         for k in ("nbr_block", "walk_window", "walk_coop", "walk_f32", "tile_width"):
             assert ctx.get_param(k) == ch[k], k
-        best = min((t for t in rep["trials"] if t["stage"] == "tiles_sph"), key=lambda t: t["ms"])
-        assert (best["tile_width"], best["unroll"]) == (ch["tile_width"], ch["unroll"])
+        # the kept tile shape is the fastest, or the starting one within 2% of it
+        ts = [t for t in rep["trials"] if t["stage"] == "tiles_sph"]
+        kept = [t for t in ts if (t["tile_width"], t["unroll"]) == (ch["tile_width"], ch["unroll"])]
+        assert len(kept) == 1 and kept[0]["ms"] <= min(t["ms"] for t in ts) * 1.02 + 1e-9
         # the kept shapes change no result
         ctx.sph(tm.FP64)
         check_fp64(ctx.results(), ref)

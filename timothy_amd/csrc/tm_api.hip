@@ -914,9 +914,13 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   if (c->stage < 3) TM_TRY(tm_step(c, p));
   std::string rep;
   char line[200];
+  // A candidate replaces the current shape only when it is faster by more
+  // than KEEP (run-to-run noise must not flip a near tie).
+  constexpr double KEEP = 0.02;
   // neighbour walk workgroup and the tree window it stages in LDS
-  int best_nb = c->nbr_block, best_win = c->walk_window;
-  double best_t = 1e30;
+  const int nb0 = c->nbr_block, win0 = c->walk_window;
+  int best_nb = nb0, best_win = win0;
+  double best_t = 1e30, cur_t = 1e30;
   const int shapes[][2] = {{64, 0}, {128, 0}, {256, 0}, {256, 512}, {256, 1024}, {256, 2048}};
   for (const auto &sh : shapes) {
     c->nbr_block = sh[0];
@@ -925,11 +929,16 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
     TM_TRY(tm_time_stage(c, TM_STAGE_NEIGHBORS, p, 1, 3, &ms));
     snprintf(line, sizeof(line), "trial neighbors nbr_block=%d walk_window=%d ms=%.4f\n", sh[0], sh[1], ms);
     rep += line;
+    if (sh[0] == nb0 && sh[1] == win0) cur_t = ms;
     if (ms < best_t) {
       best_t = ms;
       best_nb = sh[0];
       best_win = sh[1];
     }
+  }
+  if (cur_t <= best_t * (1.0 + KEEP)) {
+    best_nb = nb0;
+    best_win = win0;
   }
   c->nbr_block = best_nb;
   c->walk_window = best_win;
@@ -943,7 +952,7 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
     snprintf(line, sizeof(line), "trial neighbors walk_coop=0 ms=%.4f\ntrial neighbors walk_coop=1 walk_coop_ratio=%d ms=%.4f\n",
              tl, c->walk_coop_ratio, tc);
     rep += line;
-    c->walk_coop = tc < tl || (c0 && tc <= tl) ? 1 : 0;
+    c->walk_coop = c0 ? (tl * (1.0 + KEEP) < tc ? 0 : 1) : (tc * (1.0 + KEEP) < tl ? 1 : 0);
   }
   // that shape with the FP32 candidate screening on and off (same results)
   {
@@ -955,15 +964,16 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
       snprintf(line, sizeof(line), "trial neighbors walk_f32=%d ms=%.4f\n", c->walk_f32, t[k]);
       rep += line;
     }
-    c->walk_f32 = t[1] < t[0] ? !f0 : f0;
+    c->walk_f32 = t[1] * (1.0 + KEEP) < t[0] ? !f0 : f0;
   }
   int &unroll = (p == TM_MIXED) ? c->sph_unroll_mixed : c->sph_unroll_fp64;
   const bool tiles = c->sph_tiles && c->dim == 3;
   if (tiles) {
     // tile width x neighbours fetched per trip, timed as tile build + pairs
-    const int w0 = c->tile_width;
-    int best_w = w0, best_u = unroll;
+    const int w0 = c->tile_width, u0 = unroll;
+    int best_w = w0, best_u = u0;
     best_t = 1e30;
+    cur_t = 1e30;
     for (int w : {128, 256, 512}) {
       c->tile_width = w;
       TM_TRY(tm_run_neighbors(c, true));  // lists, and tiles of this width sized for LDS
@@ -976,12 +986,17 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
         snprintf(line, sizeof(line), "trial tiles_sph tile_width=%d unroll=%d tiles_ms=%.4f sph_ms=%.4f ms=%.4f\n", w,
                  u, tt, ms, tt + ms);
         rep += line;
+        if (w == w0 && u == u0) cur_t = tt + ms;
         if (tt + ms < best_t) {
           best_t = tt + ms;
           best_w = w;
           best_u = u;
         }
       }
+    }
+    if (cur_t <= best_t * (1.0 + KEEP)) {
+      best_w = w0;
+      best_u = u0;
     }
     if (best_w != w0) tm_graph_destroy(c);  // a captured graph holds tiles of the old width
     c->tile_width = best_w;

@@ -92,8 +92,12 @@ __device__ __forceinline__ MixedC mixed_consts(const TypeDev &T) {
 // in FP64 (so om keeps its relative accuracy where it vanishes at the
 // cutoff, the terms there scaling as om^2 and om^3) and m = 1 - 2 d/h, both
 // clamped at 0 (W = 2 om^3 - m^3 covers both branches of the M4 kernel and
-// the far stand-in gives exact zeros); sw += size_q * W in FP32, and the
-// force coefficient w / d times the FP64 separation added in FP64.
+// the far stand-in gives exact zeros); sw += size_q * W in FP32.  The force
+// coefficient w / d (w = m^2 - om^2, the kernel's slope) is formed without
+// the cancellation of m^2 - om^2 at small d: w = -(om + m) u for u <= 1/2
+// and -(om + m) om beyond (m = 0), so w / d = -min(1/h, om / d) (om + m),
+// the minimum picking the branch (om / d >= 1/h exactly when u <= 1/2); it
+// multiplies the FP64 separation, added in FP64.
 template <bool DIM3>
 __device__ __forceinline__ void pair_mixed(const double4 &P, double qx, double qy, double qz, float sq,
                                            const MixedC &K, float &sw, double &f0, double &f1, double &f2) {
