@@ -91,12 +91,12 @@ tm_status alloc_colonies(tm_ctx *c, int ncol) {
   if (ncol <= c->cap_col) return TM_OK;
   free_ptr(c->d_col_off);
   free_ptr(c->d_aff);
-  free_ptr(c->d_bbox);
+  free_ptr(c->d_col_chunk);
   free_ptr(c->d_colroot);
   c->cap_col = 0;
   if (hipMalloc((void **)&c->d_col_off, sizeof(int64_t) * (ncol + 1)) != hipSuccess ||
       hipMalloc((void **)&c->d_aff, sizeof(double4) * ncol) != hipSuccess ||
-      hipMalloc((void **)&c->d_bbox, sizeof(unsigned long long) * 6 * ncol) != hipSuccess ||
+      hipMalloc((void **)&c->d_col_chunk, sizeof(int64_t) * (ncol + 1)) != hipSuccess ||
       hipMalloc((void **)&c->d_colroot, sizeof(int32_t) * ncol) != hipSuccess)
     return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed for colony arrays");
   c->cap_col = ncol;
@@ -134,24 +134,31 @@ tm_status set_cells(tm_ctx *c, int64_t n, int32_t ncol, const int64_t *off, cons
   c->col_off = o;
   TM_HIP(c, hipMemcpyAsync(c->d_col_off, o.data(), sizeof(int64_t) * (ncol + 1), hipMemcpyHostToDevice, c->stream));
   // bounding-box chunks: each covers at most TM_BBOX_CHUNK cells of one colony
-  std::vector<int64_t> cs;
+  std::vector<int64_t> cs, cfirst;
   std::vector<int32_t> cc;
-  for (int k = 0; k < ncol; ++k)
+  for (int k = 0; k < ncol; ++k) {
+    cfirst.push_back((int64_t)cc.size());
     for (int64_t s = o[k]; s < o[k + 1]; s += TM_BBOX_CHUNK) {
       cs.push_back(s);
       cc.push_back(k);
     }
+  }
+  cfirst.push_back((int64_t)cc.size());
   cs.push_back(n);
   c->nchunks = (int64_t)cc.size();
   if (c->nchunks + 1 > c->cap_chunks) {
     free_ptr(c->d_chunk_start);
     free_ptr(c->d_chunk_cid);
+    free_ptr(c->d_cbox);
     c->cap_chunks = 0;
     if (hipMalloc((void **)&c->d_chunk_start, sizeof(int64_t) * (c->nchunks + 1)) != hipSuccess ||
-        hipMalloc((void **)&c->d_chunk_cid, sizeof(int32_t) * (c->nchunks + 1)) != hipSuccess)
+        hipMalloc((void **)&c->d_chunk_cid, sizeof(int32_t) * (c->nchunks + 1)) != hipSuccess ||
+        hipMalloc((void **)&c->d_cbox, sizeof(unsigned long long) * 6 * (c->nchunks + 1)) != hipSuccess)
       return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed for bounding-box chunks");
     c->cap_chunks = c->nchunks + 1;
   }
+  TM_HIP(c, hipMemcpyAsync(c->d_col_chunk, cfirst.data(), sizeof(int64_t) * cfirst.size(), hipMemcpyHostToDevice,
+                           c->stream));
   TM_HIP(c, hipMemcpyAsync(c->d_chunk_start, cs.data(), sizeof(int64_t) * cs.size(), hipMemcpyHostToDevice,
                            c->stream));
   if (c->nchunks > 0)
@@ -321,7 +328,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_scan);
   free_ptr(c->d_col_off);
   free_ptr(c->d_aff);
-  free_ptr(c->d_bbox);
+  free_ptr(c->d_col_chunk);
   free_ptr(c->d_colroot);
   free_ptr(c->own_x);
   free_ptr(c->own_y);
@@ -338,6 +345,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_aux_o);
   free_ptr(c->d_chunk_start);
   free_ptr(c->d_chunk_cid);
+  free_ptr(c->d_cbox);
   free_ptr(c->d_tile_idx);
   free_ptr(c->d_tile_cnt);
   free_ptr(c->d_tile_cell);

@@ -76,20 +76,17 @@ __device__ __forceinline__ int key_delta(const uint4 *__restrict__ code, int64_t
   return TM_CID_BITS + 3 * (28 - bit) + r;
 }
 
-__global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v0, unsigned long long v1, int per) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = ((i % per) < 3) ? v0 : v1;
-}
 
 // bounding box of x -+ (h_type + 0.01) per colony (src/octree.c:161-174): one
-// workgroup per chunk of one colony, reduced in registers and LDS, then six
-// atomics per workgroup on order-preserving integer images of the doubles
+// workgroup per chunk of one colony, reduced in registers and LDS into the
+// chunk's six order-preserving integer images of the doubles (k_affine
+// reduces a colony's chunks; no atomics on six shared words)
 __global__ __launch_bounds__(BT) void k_bbox(const double *__restrict__ x, const double *__restrict__ y,
                                              const double *__restrict__ z, const int32_t *__restrict__ ctype,
                                              const int64_t *__restrict__ chunk_start,
                                              const int32_t *__restrict__ chunk_cid,
                                              const TypeDev *__restrict__ types, int ntypes,
-                                             unsigned long long *__restrict__ bbox, unsigned int *__restrict__ flags) {
+                                             unsigned long long *__restrict__ cbox, unsigned int *__restrict__ flags) {
   __shared__ unsigned long long red[BT / 64][6];
   const int64_t s = chunk_start[blockIdx.x], e = chunk_start[blockIdx.x + 1];
   unsigned long long k[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
@@ -129,23 +126,40 @@ __global__ __launch_bounds__(BT) void k_bbox(const double *__restrict__ x, const
         k[a] = red[q][a] < k[a] ? red[q][a] : k[a];
         k[a + 3] = red[q][a + 3] > k[a + 3] ? red[q][a + 3] : k[a + 3];
       }
-    unsigned long long *b = bbox + 6 * (int64_t)chunk_cid[blockIdx.x];
-    for (int a = 0; a < 3; ++a) {
-      atomicMin(b + a, k[a]);
-      atomicMax(b + 3 + a, k[a + 3]);
-    }
+    unsigned long long *b = cbox + 6 * (int64_t)blockIdx.x;
+    for (int a = 0; a < 6; ++a) b[a] = k[a];
   }
 }
 
-// src/octree.c:175-180
-__global__ void k_affine(const unsigned long long *__restrict__ bbox, int ncol, double4 *__restrict__ aff) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+// src/octree.c:175-180: one wave per colony reduces its chunks' boxes
+__global__ __launch_bounds__(64) void k_affine(const unsigned long long *__restrict__ cbox,
+                                               const int64_t *__restrict__ col_chunk, int ncol,
+                                               double4 *__restrict__ aff) {
+  const int c = blockIdx.x;
   if (c >= ncol) return;
-  const unsigned long long *b = bbox + 6 * (int64_t)c;
+  unsigned long long k[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+  for (int64_t q = col_chunk[c] + threadIdx.x; q < col_chunk[c + 1]; q += 64) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const unsigned long long lo = cbox[6 * q + a], hi = cbox[6 * q + 3 + a];
+      k[a] = lo < k[a] ? lo : k[a];
+      k[a + 3] = hi > k[a + 3] ? hi : k[a + 3];
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const unsigned long long t = __shfl_xor(k[a], o, 64), u = __shfl_xor(k[a + 3], o, 64);
+      k[a] = t < k[a] ? t : k[a];
+      k[a + 3] = u > k[a + 3] ? u : k[a + 3];
+    }
+  }
+  if (threadIdx.x != 0) return;
   double mn[3], mx[3];
   for (int a = 0; a < 3; ++a) {
-    mn[a] = unord_key(b[a]);
-    mx[a] = unord_key(b[3 + a]);
+    mn[a] = unord_key(k[a]);
+    mx[a] = unord_key(k[3 + a]);
   }
   double s = mx[0] - mn[0];
   s = (s > mx[1] - mn[1] ? s : mx[1] - mn[1]);
@@ -469,12 +483,10 @@ tm_status tm_run_build(tm_ctx *c) {
   hipStream_t st = c->stream;
   k_zero_u32<<<1, 64, 0, st>>>(c->d_flags, 4);
   // bounding boxes and affine maps
-  int64_t nb6 = 6 * (int64_t)c->ncol;
-  k_fill_u64<<<tm_blocks(nb6, 256), 256, 0, st>>>(c->d_bbox, nb6, ~0ull, 0ull, 6);
   if (c->nchunks > 0)
     k_bbox<<<(unsigned)c->nchunks, BT, 0, st>>>(c->x, c->y, c->z, c->ctype, c->d_chunk_start, c->d_chunk_cid,
-                                                 c->d_types, c->ntypes, c->d_bbox, c->d_flags);
-  k_affine<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_bbox, c->ncol, c->d_aff);
+                                                 c->d_types, c->ntypes, c->d_cbox, c->d_flags);
+  k_affine<<<(unsigned)c->ncol, 64, 0, st>>>(c->d_cbox, c->d_col_chunk, c->ncol, c->d_aff);
   TM_HIP(c, hipGetLastError());
   // Morton keys and the sort
   int cb = 0;

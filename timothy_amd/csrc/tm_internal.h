@@ -16,7 +16,7 @@
 #define TM_FLAG_ROOT 2
 #define TM_FLAG_CTYPE 4
 #define TM_FLAG_WALK 8
-#define TM_BBOX_CHUNK 8192
+#define TM_BBOX_CHUNK 2048
 // LDS tiles: the pair kernels take cells in tiles of tile_width (128, 256 or
 // 512; TM_TILE by default) consecutive (Morton-order) cells; the distinct
 // neighbours of a tile, at most
@@ -103,13 +103,14 @@ struct tm_ctx {
   int64_t nchunks = 0, cap_chunks = 0;
   int64_t *d_chunk_start = nullptr;  // nchunks + 1
   int32_t *d_chunk_cid = nullptr;
+  unsigned long long *d_cbox = nullptr;  // per chunk 6 ordered keys: its cells' bounding box
   // inputs packed in reference order by the key kernel
   double4 *d_pos_o = nullptr;  // x, y, z, size
   uint4 *d_aux_o = nullptr;    // young bits, ctype, phase, colony
 
   // per colony
   double4 *d_aff = nullptr;              // shift.xyz, scale
-  unsigned long long *d_bbox = nullptr;  // 6 ordered keys per colony
+  int64_t *d_col_chunk = nullptr;  // per colony its first bounding-box chunk (ncol + 1)
   int32_t *d_colroot = nullptr;
 
   // sort scratch
