@@ -41,7 +41,6 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
   free_ptr(c->d_ctype_s);
   free_ptr(c->d_phase_s);
   free_ptr(c->d_span);
-  free_ptr(c->d_nodes);
   free_ptr(c->d_wn);
   free_ptr(c->d_wpar);
   free_ptr(c->d_range);
@@ -70,7 +69,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
       {(void **)&c->d_key, 8},      {(void **)&c->d_key_alt, 8},  {(void **)&c->d_val_alt, 4},
       {(void **)&c->d_perm, 4},     {(void **)&c->d_code, 16},    {(void **)&c->d_pos, 32},    {(void **)&c->d_posf, 16},
       {(void **)&c->d_young_s, 4},  {(void **)&c->d_ctype_s, 4},
-      {(void **)&c->d_phase_s, 4},  {(void **)&c->d_span, 4},     {(void **)&c->d_nodes, 64},
+      {(void **)&c->d_phase_s, 4},  {(void **)&c->d_span, 4},
       {(void **)&c->d_wn, 32},      {(void **)&c->d_wpar, 8},     {(void **)&c->d_range, 8},    {(void **)&c->d_delta, 4},    {(void **)&c->d_ncount, 4},
       {(void **)&c->d_qstar, 4},    {(void **)&c->d_dens, 8},     {(void **)&c->d_v, 8},
       {(void **)&c->d_mind, 8},     {(void **)&c->d_fx, 8},       {(void **)&c->d_fy, 8},
@@ -309,7 +308,6 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_ctype_s);
   free_ptr(c->d_phase_s);
   free_ptr(c->d_span);
-  free_ptr(c->d_nodes);
   free_ptr(c->d_wn);
   free_ptr(c->d_wpar);
   free_ptr(c->d_range);

@@ -300,8 +300,7 @@ __global__ __launch_bounds__(BT) void k_leaf(const uint4 *__restrict__ code, int
 // range, common-prefix length, region and the child the descending walk
 // visits first (the right one).
 __global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, int64_t n, int2 *__restrict__ range,
-                                               int32_t *__restrict__ delta, int32_t *__restrict__ split,
-                                               NodeRec *__restrict__ nodes) {
+                                               int32_t *__restrict__ delta, int32_t *__restrict__ split) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n - 1) return;
   int d = (key_delta(code, n, i, i + 1) - key_delta(code, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -319,33 +318,9 @@ __global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, i
     if (key_delta(code, n, i, i + (s + t) * d) > dnode) s += t;
   } while (t > 1);
   int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
-  int64_t first = min(i, j), last = max(i, j);
-  range[i] = make_int2((int)first, (int)last);
+  range[i] = make_int2((int)min(i, j), (int)max(i, j));
   delta[i] = dnode;
   split[i] = (int32_t)gamma;
-  NodeRec r;
-  uint4 c0 = code[first];
-  if (dnode >= TM_CID_BITS) {
-    int dm = dnode - TM_CID_BITS;
-    int k = dm / 3, rr = dm % 3;
-    int nb[3] = {k, k + (rr >= 2 ? 1 : 0), k + (rr >= 1 ? 1 : 0)};  // x, y, z fixed bits
-    uint32_t cc[3] = {c0.x, c0.y, c0.z};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      uint32_t side = 1u << (29 - nb[a]);
-      r.lo[a] = cc[a] & ~(side - 1u);
-      r.hi[a] = r.lo[a] + side;
-    }
-  } else {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      r.lo[a] = 0;
-      r.hi[a] = 1u << 29;
-    }
-  }
-  r.child = (last == gamma + 1) ? (uint32_t)((n - 1) + gamma + 1) : (uint32_t)(gamma + 1);
-  r.esc = TM_END;
-  nodes[i] = r;
 }
 
 __global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 *__restrict__ range, int64_t n,
@@ -504,7 +479,7 @@ tm_status tm_run_build(tm_ctx *c) {
   TM_HIP(c, hipGetLastError());
   // radix tree
   if (n > 1)
-    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, n, c->d_range, c->d_delta, c->d_split, c->d_nodes);
+    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, n, c->d_range, c->d_delta, c->d_split);
   k_colroot<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_col_off, c->ncol, c->d_range, n, c->d_colroot, c->d_flags);
   if (n > 1) k_wparents<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_range, c->d_split, n, c->d_wpar);
   k_wnodes<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_wpar,

@@ -52,17 +52,6 @@ struct TypeDev {
   float hinv_f;
 };
 
-// Internal node of the radix tree in key order: its region [lo, hi) in
-// location-code units per axis and the child the descending walk visits
-// first.  The reference-layout octree is emitted from these; the walk reads
-// the pre-ordered copy (d_wn).
-struct NodeRec {
-  uint32_t lo[3];
-  uint32_t child;
-  uint32_t hi[3];
-  uint32_t esc;
-};
-
 struct tm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -129,7 +118,6 @@ struct tm_ctx {
   int32_t *d_span = nullptr;
 
   // radix tree
-  NodeRec *d_nodes = nullptr;
   // the walk's tree: every colony's nodes in its descending pre-order (see
   // tm_build.hip), and the parent links they are placed from
   uint4 *d_wn = nullptr;

@@ -55,14 +55,17 @@ __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
 }
 
 // parent of every radix node (internal: k, leaves: n-1+i)
-__global__ void k_parents(const int2 *__restrict__ range, const int32_t *__restrict__ split,
-                          const NodeRec *__restrict__ nodes, int64_t n, int32_t *__restrict__ parent) {
+__global__ void k_parents(const int2 *__restrict__ range, const int32_t *__restrict__ split, int64_t n,
+                          int32_t *__restrict__ parent) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n - 1) return;
-  int32_t g = split[k];
-  int64_t left = (range[k].x == g) ? (n - 1) + g : g;
+  const int32_t g = split[k];
+  const int2 r = range[k];
+  // children [first, g] and [g + 1, last]: a leaf (n - 1 + cell) when one cell
+  const int64_t left = (r.x == g) ? (n - 1) + g : g;
+  const int64_t right = (r.y == g + 1) ? (n - 1) + g + 1 : g + 1;
   parent[left] = (int32_t)k;
-  parent[nodes[k].child] = (int32_t)k;
+  parent[right] = (int32_t)k;
 }
 
 __global__ void k_chain_counts(const int2 *__restrict__ range, const int32_t *__restrict__ delta,
@@ -181,7 +184,7 @@ tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count) {
   TM_HIP(c, hipStreamSynchronize(c->stream));
   if (root < 0 || root > n - 2) return tm_fail(c, TM_ERR_STATE, "colony has no internal root");
   k_fill_i32<<<tm_blocks(2 * n - 1, 256), 256, 0, c->stream>>>(c->d_parent, 2 * n - 1, -1);
-  k_parents<<<tm_blocks(n - 1, 256), 256, 0, c->stream>>>(c->d_range, c->d_split, c->d_nodes, n, c->d_parent);
+  k_parents<<<tm_blocks(n - 1, 256), 256, 0, c->stream>>>(c->d_range, c->d_split, n, c->d_parent);
   uint32_t *cnt = nullptr;
   TM_HIP(c, hipMalloc((void **)&cnt, sizeof(uint32_t) * (size_t)(nc + 1)));
   k_chain_counts<<<tm_blocks(nc, 256), 256, 0, c->stream>>>(c->d_range, c->d_delta, c->d_parent, n, s, e, root, cnt);
