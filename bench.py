@@ -164,6 +164,36 @@ def compat_step(col, reps=3):
     return out
 
 
+def direct_path(tm, ctx, a, build_ms, cic_ms):
+    """the same step without LDS tiles (sph_tiles = 0): neighbour lists
+    without the tile build, the direct pair kernels at their best shape
+    (workgroup x records per trip x lanes per cell); the context is left as
+    it was"""
+    keep = {k: ctx.get_param(k) for k in ("sph_tiles", "sph_block", "sph_lanes", "sph_unroll_mixed",
+                                          "sph_unroll_fp64")}
+    ctx.set_param("sph_tiles", 0)
+    ctx.neighbors()
+    nbr = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
+    best = {}
+    for prec, name, key in ((tm.MIXED, "mixed", "sph_unroll_mixed"), (tm.FP64, "fp64", "sph_unroll_fp64")):
+        for lanes in ((1, 2, 4, 8) if prec == tm.MIXED else (1,)):
+            for bs in (64, 128, 256):
+                for u in (1, 2, 4, 8):
+                    ctx.set_param("sph_lanes", lanes)
+                    ctx.set_param("sph_block", bs)
+                    ctx.set_param(key, u)
+                    ms = ctx.time_stage(tm.STAGE_SPH, prec, warmup=3, iters=20)
+                    if name not in best or ms < best[name]["ms"]:
+                        best[name] = {"ms": ms, "lanes": lanes, "sph_block": bs, "unroll": u}
+    for k, v in keep.items():
+        ctx.set_param(k, v)
+    ctx.neighbors()
+    return {"neighbors_ms": nbr, "sph_mixed": best["mixed"], "sph_fp64": best["fp64"],
+            "full_step_ms_mixed": build_ms + nbr + best["mixed"]["ms"] + cic_ms,
+            "note": "sph_tiles = 0: the tiles make the pair kernel faster but cost more to build than that saves "
+                    "in a step at this size"}
+
+
 def gpu_pairs(tm, col):
     with tm.Context(0) as c:
         c.upload_colony(col)
@@ -348,6 +378,7 @@ def run_single(a, tm, gen):
     nbr_ms = ctx.time_stage(tm.STAGE_NEIGHBORS, tm.MIXED, warmup=1, iters=3)
     cic_ms = ctx.time_stage(tm.STAGE_CIC, tm.MIXED, warmup=1, iters=3)
     tiles_ms = ctx.time_stage(tm.STAGE_TILES, tm.MIXED, warmup=2, iters=10)
+    direct = direct_path(tm, ctx, a, build_ms, cic_ms)
     env = environment_solve(tm, ctx, grid, prod)
     prod.free()
     copy_gbs = copy_bandwidth(tm, ctx)
@@ -413,6 +444,7 @@ def run_single(a, tm, gen):
         "full_step_ms": {"total": gpu_full, "build": build_ms, "neighbors": nbr_ms, "tiles_in_neighbors": tiles_ms,
                          "sph_mixed": sph_mixed, "cic_deposit_256^3": cic_ms},
         "like_for_like": lfl,
+        "without_tiles": direct,
         "mechanics_loop_ms_per_step": loop_ms,
         "compat_step_host_arrays": compat,
         "environment_solve_256^3": env,
