@@ -406,6 +406,24 @@ def test_cic_deposit_grid_sequence(tm, oracle_mod, ctx):
         say(f"  grid {k} ls={ls.tolist()} mode {mode}: {int((ref != 0).sum())} nodes, "
             f"max rel {np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)):.2e}")
         prod.free()
+    # every cell necrotic (phase 5, deposits nothing), and a grid that
+    # holds no cell: all nodes written as zeros
+    ph = np.asarray(col["phase"]).copy()
+    dead = dict(col, phase=np.full(col["n"], 5, np.int32))
+    ctx.upload_colony(dead)
+    ctx.build()
+    far_blk = block([200, 200, 200], [231, 215, 247])
+    for grid, mode in ((g0, 0), (far_blk, 0), (far_blk, 1)):
+        prod = tm.DeviceArray(ctx, int(np.prod(grid["localsize"])), np.float64)
+        prod.set(np.full(int(np.prod(grid["localsize"])), 7.0))
+        ctx.cic_deposit(grid, prod, mode)
+        got = prod.get()
+        ref = O.cic_deposit(O.Colony.from_dict(dead), grid, mode)
+        assert np.array_equal(got, ref) and (mode == 1 or not ref.any()), mode
+        prod.free()
+    assert np.array_equal(np.asarray(col["phase"]), ph)
+    ctx.upload_colony(col)
+    ctx.build()
     # node indices outside (-2^29, 2^29) are refused before any launch
     far = block([-(1 << 29), 0, 0], [-(1 << 29) + 3, 3, 3])
     prod = tm.DeviceArray(ctx, 64, np.float64)
