@@ -76,6 +76,20 @@ __device__ __forceinline__ int key_delta(const uint4 *__restrict__ code, int64_t
   return TM_CID_BITS + 3 * (28 - bit) + r;
 }
 
+// The same prefix length from the sorted 64-bit sort keys where they differ
+// (cb top bits the colony id, then the Morton key's leading 64 - cb bits,
+// z before y before x in each triple as in the code comparison above), the
+// codes only where the keys tie: half the bytes per probe of the searches.
+__device__ __forceinline__ int key_delta64(const uint64_t *__restrict__ key, uint64_t ki, const uint4 *__restrict__ code,
+                                           int64_t n, int cb, int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t x = ki ^ key[j];
+  if (x) {
+    const int p = __clzll((long long)x);
+    return p < cb ? TM_CID_BITS - cb + p : TM_CID_BITS + p - cb;
+  }
+  return key_delta(code, n, i, j);
+}
 
 // bounding box of x -+ (h_type + 0.01) per colony (src/octree.c:161-174): one
 // workgroup per chunk of one colony, reduced in registers and LDS into the
@@ -299,23 +313,26 @@ __global__ __launch_bounds__(BT) void k_leaf(const uint4 *__restrict__ code, int
 // Karras (2012) radix tree over (colony, Morton) keys: internal node i's key
 // range, common-prefix length, region and the child the descending walk
 // visits first (the right one).
-__global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, int64_t n, int2 *__restrict__ range,
+__global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, const uint64_t *__restrict__ key,
+                                               int64_t n, int cb, int2 *__restrict__ range,
                                                int32_t *__restrict__ delta, int32_t *__restrict__ split) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n - 1) return;
-  int d = (key_delta(code, n, i, i + 1) - key_delta(code, n, i, i - 1)) >= 0 ? 1 : -1;
-  int dmin = key_delta(code, n, i, i - d);
+  const uint64_t ki = key[i];
+  auto dl = [&](int64_t j) { return key_delta64(key, ki, code, n, cb, i, j); };
+  int d = (dl(i + 1) - dl(i - 1)) >= 0 ? 1 : -1;
+  int dmin = dl(i - d);
   int64_t lmax = 2;
-  while (key_delta(code, n, i, i + lmax * d) > dmin) lmax *= 2;
+  while (dl(i + lmax * d) > dmin) lmax *= 2;
   int64_t l = 0;
   for (int64_t t = lmax / 2; t >= 1; t /= 2)
-    if (key_delta(code, n, i, i + (l + t) * d) > dmin) l += t;
+    if (dl(i + (l + t) * d) > dmin) l += t;
   int64_t j = i + l * d;
-  int dnode = key_delta(code, n, i, j);
+  int dnode = dl(j);
   int64_t s = 0, t = l;
   do {
     t = (t + 1) >> 1;
-    if (key_delta(code, n, i, i + (s + t) * d) > dnode) s += t;
+    if (dl(i + (s + t) * d) > dnode) s += t;
   } while (t > 1);
   int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
   range[i] = make_int2((int)min(i, j), (int)max(i, j));
@@ -479,7 +496,7 @@ tm_status tm_run_build(tm_ctx *c) {
   TM_HIP(c, hipGetLastError());
   // radix tree
   if (n > 1)
-    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, n, c->d_range, c->d_delta, c->d_split);
+    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, c->d_key, n, cb, c->d_range, c->d_delta, c->d_split);
   k_colroot<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_col_off, c->ncol, c->d_range, n, c->d_colroot, c->d_flags);
   if (n > 1) k_wparents<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_range, c->d_split, n, c->d_wpar);
   k_wnodes<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_wpar,
