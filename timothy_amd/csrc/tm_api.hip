@@ -42,7 +42,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
   free_ptr(c->d_phase_s);
   free_ptr(c->d_span);
   free_ptr(c->d_wn);
-  free_ptr(c->d_wpar);
+  free_ptr(c->d_wchain);
   free_ptr(c->d_range);
   free_ptr(c->d_delta);
   free_ptr(c->d_ncount);
@@ -70,7 +70,7 @@ tm_status alloc_cells(tm_ctx *c, int64_t n) {
       {(void **)&c->d_perm, 4},     {(void **)&c->d_code, 16},    {(void **)&c->d_pos, 32},    {(void **)&c->d_posf, 16},
       {(void **)&c->d_young_s, 4},  {(void **)&c->d_ctype_s, 4},
       {(void **)&c->d_phase_s, 4},  {(void **)&c->d_span, 4},
-      {(void **)&c->d_wn, 32},      {(void **)&c->d_wpar, 8},     {(void **)&c->d_range, 8},    {(void **)&c->d_delta, 4},    {(void **)&c->d_ncount, 4},
+      {(void **)&c->d_wn, 32},      {(void **)&c->d_wchain, 4},     {(void **)&c->d_range, 8},    {(void **)&c->d_delta, 4},    {(void **)&c->d_ncount, 4},
       {(void **)&c->d_qstar, 4},    {(void **)&c->d_dens, 8},     {(void **)&c->d_v, 8},
       {(void **)&c->d_mind, 8},     {(void **)&c->d_fx, 8},       {(void **)&c->d_fy, 8},
       {(void **)&c->d_fz, 8},       {(void **)&c->d_out, 48},     {(void **)&c->d_split, 4},
@@ -309,7 +309,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_phase_s);
   free_ptr(c->d_span);
   free_ptr(c->d_wn);
-  free_ptr(c->d_wpar);
+  free_ptr(c->d_wchain);
   free_ptr(c->d_range);
   free_ptr(c->d_delta);
   free_ptr(c->d_ncount);

@@ -282,9 +282,10 @@ __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key
 // below node level 3, where the reference shifts by negative counts
 // (src/octree.c:105-107).
 __global__ __launch_bounds__(BT) void k_leaf(const uint4 *__restrict__ code, int64_t n, int32_t *__restrict__ span,
-                                             unsigned int *__restrict__ flags) {
+                                             uint32_t *__restrict__ chain, unsigned int *__restrict__ flags) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n) return;
+  chain[i] = 0;  // counted by k_karras
   uint4 me = code[i];
   int best = 99;
   bool bad = false;
@@ -311,11 +312,12 @@ __global__ __launch_bounds__(BT) void k_leaf(const uint4 *__restrict__ code, int
 }
 
 // Karras (2012) radix tree over (colony, Morton) keys: internal node i's key
-// range, common-prefix length, region and the child the descending walk
-// visits first (the right one).
+// range, common-prefix length and split.  chain[v] counts the nodes inside
+// one colony (common prefix at least the colony id) whose range ends at v.
 __global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, const uint64_t *__restrict__ key,
                                                int64_t n, int cb, int2 *__restrict__ range,
-                                               int32_t *__restrict__ delta, int32_t *__restrict__ split) {
+                                               int32_t *__restrict__ delta, int32_t *__restrict__ split,
+                                               uint32_t *__restrict__ chain) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n - 1) return;
   const uint64_t ki = key[i];
@@ -338,6 +340,7 @@ __global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, c
   range[i] = make_int2((int)min(i, j), (int)max(i, j));
   delta[i] = dnode;
   split[i] = (int32_t)gamma;
+  if (dnode >= TM_CID_BITS) atomicAdd(&chain[max(i, j)], 1u);
 }
 
 __global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 *__restrict__ range, int64_t n,
@@ -373,17 +376,6 @@ __global__ void k_colroot(const int64_t *__restrict__ off, int ncol, const int2 
 // the x side; bit 31 of y set for a leaf) and, in .w,
 // the escape slot of an internal node or the cell of a leaf (whose escape is
 // always the next slot).
-__global__ __launch_bounds__(BT) void k_wparents(const int2 *__restrict__ range, const int32_t *__restrict__ split,
-                                                 int64_t n, int32_t *__restrict__ par) {
-  const int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
-  if (k >= n - 1) return;
-  const int2 r = range[k];
-  const int32_t g = split[k];
-  const int64_t left = (r.x == g) ? (n - 1) + g : g;
-  const int64_t right = (r.y == g + 1) ? (n - 1) + g + 1 : g + 1;
-  par[left] = (int32_t)(k << 1);
-  par[right] = (int32_t)((k << 1) | 1);  // the child the descending walk visits first
-}
 
 // the record of node k (a leaf when k >= n - 1) covering cells [f, l], placed
 // at slot pos
@@ -414,46 +406,39 @@ __device__ __forceinline__ void put_wnode(const uint4 *__restrict__ code, const 
   wn[pos] = make_uint4(lx | ((L & 7u) << 29), ly | ((L >> 3) << 29) | (leaf ? 0x80000000u : 0u), lz | (halved << 29), w);
 }
 
-// One thread per cell l.  The nodes whose cells end at l are the leaf l and
-// the chain above it reached through first-visited links (a parent's first
-// child holds its highest keys, so it ends where the parent ends); each
-// node is on exactly one such chain.  The thread counts the first-visited
-// links from the leaf up to the colony root once, then places the leaf and
-// its chain, each node one such link fewer than the one below it.
+// One thread per node k (a leaf when k >= n - 1) covering cells [f, l] of
+// colony [s, e]; internal nodes above the colony roots are not placed.  The
+// ancestors of the node in whose first-visited child it lies are the
+// colony's internal nodes B with split(B) < f and l <= last(B): the
+// colony's boundaries (p - 1, p) for s < p <= f are each the split of one
+// internal node of the colony, and the ranges nest, so of those nodes the
+// ones that are not such ancestors are the ones ending before f -- the
+// P[f] - P[s] nodes counted at cells s .. f - 1 (P the exclusive sums of the
+// chain counts).  So the node sits at
+//   2s + 2 (e - l) + (f - s) - (P[f] - P[s]).
 __global__ __launch_bounds__(BT) void k_wnodes(const uint4 *__restrict__ code, const int32_t *__restrict__ span,
                                                const int2 *__restrict__ range, const int32_t *__restrict__ delta,
-                                               const int32_t *__restrict__ par, const int64_t *__restrict__ off,
-                                               const int32_t *__restrict__ colroot, int64_t n,
-                                               uint4 *__restrict__ wn, unsigned int *__restrict__ flags) {
-  const int64_t l = (int64_t)blockIdx.x * BT + threadIdx.x;
-  if (l >= n) return;
-  const uint32_t cid = code[l].w;
+                                               const int64_t *__restrict__ off, const uint32_t *__restrict__ P,
+                                               int64_t n, uint4 *__restrict__ wn, unsigned int *__restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * BT + threadIdx.x;
+  if (k >= 2 * n - 1) return;
+  int64_t f, l;
+  if (k < n - 1) {
+    if (delta[k] < TM_CID_BITS) return;
+    const int2 r = range[k];
+    f = r.x;
+    l = r.y;
+  } else {
+    f = l = k - (n - 1);
+  }
+  const uint32_t cid = code[f].w;
   const int64_t s = off[cid], e = off[cid + 1] - 1;
-  const int64_t root = colroot[cid];
-  const int64_t leafk = (n - 1) + l;
-  int64_t cur = leafk;
-  uint32_t first = 0, chain = 0;
-  bool in_chain = true;
-  for (int depth = 0; cur != root; ++depth) {
-    if (depth > 256 || cur < 0 || cur >= 2 * n - 1) {  // deeper than any key allows: a malformed tree
-      atomicOr(flags, (unsigned)TM_FLAG_WALK);
-      return;
-    }
-    const int32_t p = par[cur];
-    const uint32_t fv = (uint32_t)(p & 1);
-    first += fv;
-    in_chain = in_chain && fv;
-    chain += in_chain ? 1u : 0u;
-    cur = p >> 1;
+  const int64_t pos = 2 * s + 2 * (e - l) + (f - s) - (int64_t)(P[f] - P[s]);
+  if (l > e || pos < 2 * s || pos > 2 * e) {  // a malformed tree: a slot outside the colony's
+    atomicOr(flags, (unsigned)TM_FLAG_WALK);
+    return;
   }
-  const int64_t base = 2 * s + 2 * (e - l);
-  put_wnode(code, span, delta, n, leafk, l, l, base + first, wn);
-  cur = leafk;
-  for (uint32_t c = 1; c <= chain; ++c) {
-    cur = par[cur] >> 1;
-    const int2 r = range[cur];
-    put_wnode(code, span, delta, n, cur, r.x, l, base + first - c, wn);
-  }
+  put_wnode(code, span, delta, n, k, f, l, pos, wn);
 }
 
 __global__ void k_zero_u32(unsigned int *p, int n) {
@@ -492,15 +477,16 @@ tm_status tm_run_build(tm_ctx *c) {
                                              c->d_posf, c->d_young_s, c->d_ctype_s, c->d_phase_s);
   k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_posf,
                                               c->d_young_s, c->d_ctype_s, c->d_phase_s, c->d_flags);
-  k_leaf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, n, c->d_span, c->d_flags);
+  k_leaf<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, n, c->d_span, c->d_wchain, c->d_flags);
   TM_HIP(c, hipGetLastError());
   // radix tree
   if (n > 1)
-    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, c->d_key, n, cb, c->d_range, c->d_delta, c->d_split);
+    k_karras<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_code, c->d_key, n, cb, c->d_range, c->d_delta, c->d_split,
+                                                   c->d_wchain);
   k_colroot<<<tm_blocks(c->ncol, 64), 64, 0, st>>>(c->d_col_off, c->ncol, c->d_range, n, c->d_colroot, c->d_flags);
-  if (n > 1) k_wparents<<<tm_blocks(n - 1, BT), BT, 0, st>>>(c->d_range, c->d_split, n, c->d_wpar);
-  k_wnodes<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_wpar,
-                                                    c->d_col_off, c->d_colroot, n, c->d_wn, c->d_flags);
+  TM_TRY(tm_scan_u32(c, c->d_wchain, c->d_wchain, n));
+  k_wnodes<<<tm_blocks(2 * n - 1, BT), BT, 0, st>>>(c->d_code, c->d_span, c->d_range, c->d_delta, c->d_col_off,
+                                                     c->d_wchain, n, c->d_wn, c->d_flags);
   TM_HIP(c, hipGetLastError());
   c->stage = 2;
   return TM_OK;

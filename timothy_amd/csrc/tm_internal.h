@@ -119,9 +119,9 @@ struct tm_ctx {
 
   // radix tree
   // the walk's tree: every colony's nodes in its descending pre-order (see
-  // tm_build.hip), and the parent links they are placed from
+  // tm_build.hip), and the counts their slots are computed from
   uint4 *d_wn = nullptr;
-  int32_t *d_wpar = nullptr;
+  uint32_t *d_wchain = nullptr;  // per cell: the colony's internal nodes ending there, then their exclusive sums
   int2 *d_range = nullptr;
   int32_t *d_delta = nullptr;
   int32_t *d_split = nullptr;   // split position of each internal node
