@@ -210,6 +210,11 @@ tm_status tm_free(tm_ctx *ctx, void *ptr);
 tm_status tm_memcpy_htod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 tm_status tm_memcpy_dtoh(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 tm_status tm_memset(tm_ctx *ctx, void *dst, int value, size_t bytes);
+/* page-locked host memory: copies between it and the device run at the
+ * full link rate (the compat layer stages cells and the emitted octree in
+ * it).  tm_host_free takes only pointers tm_host_alloc returned. */
+tm_status tm_host_alloc(tm_ctx *ctx, void **ptr, size_t bytes);
+tm_status tm_host_free(tm_ctx *ctx, void *ptr);
 /* device-to-device copy on the context's stream (also the known-size
  * reference transfer used to calibrate memory-traffic counters) */
 tm_status tm_memcpy_dtod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);

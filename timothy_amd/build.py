@@ -63,7 +63,7 @@ def build(verbose: bool = False) -> None:
     if not os.path.exists(os.path.join(CSRC, "compat", "tm_compat.c")):
         return
     tmp = COMPAT + ".tmp"
-    _run(["gcc", "-std=gnu11", "-O2", "-fPIC", "-shared", "-Wall", "-Wextra", "-Wno-unused-parameter",
+    _run(["gcc", "-std=gnu11", "-O2", "-fPIC", "-shared", "-fopenmp", "-Wall", "-Wextra", "-Wno-unused-parameter",
           "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),
           os.path.join(CSRC, "compat", "tm_compat.c"), "-o", tmp,
           "-L" + HERE, "-ltimothy_hip", "-Wl,-rpath,$ORIGIN", "-lm"])

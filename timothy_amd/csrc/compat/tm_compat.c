@@ -48,6 +48,12 @@ static struct {
   float *h_young;
   int32_t *h_ctype, *h_phase;
   int64_t h_cap;
+  void *h_block; /* page-locked block holding the arrays above */
+  /* the emitted octree: page-locked, kept between steps (handed to the
+   * caller as cellsinfo->octree by octbuild, taken back by octfree) */
+  octnode_t *oct_buf;
+  int64_t oct_cap;
+  int oct_out;
   /* grid transfer buffers */
   double *d_prod, *d_field, *d_grad, *d_interp;
   int64_t grid_cap, interp_cap;
@@ -135,21 +141,23 @@ void tm_compat_set_host_octree(int enable) {
 void *tm_compat_context(void) { return S.ctx; }
 
 static void free_host(void) {
-  free(S.h_x);
-  free(S.h_y);
-  free(S.h_z);
-  free(S.h_size);
-  free(S.h_out);
-  free(S.h_young);
-  free(S.h_ctype);
-  free(S.h_phase);
+  if (S.h_block && S.ctx) tm_host_free(S.ctx, S.h_block);
+  S.h_block = NULL;
   S.h_x = S.h_y = S.h_z = S.h_size = S.h_out = NULL;
   S.h_young = NULL;
   S.h_ctype = S.h_phase = NULL;
   S.h_cap = 0;
+  /* the octree buffer, unless a caller still holds it (octfree then takes it
+   * back; it lives on until the process ends) */
+  if (S.oct_buf && S.ctx && !S.oct_out) {
+    tm_host_free(S.ctx, S.oct_buf);
+    S.oct_buf = NULL;
+    S.oct_cap = 0;
+  }
 }
 
 void tm_compat_shutdown(void) {
+  free_host();
   if (S.ctx) {
     free_env();
     if (S.d_prod) tm_free(S.ctx, S.d_prod);
@@ -157,6 +165,7 @@ void tm_compat_shutdown(void) {
     if (S.d_grad) tm_free(S.ctx, S.d_grad);
     if (S.d_interp) tm_free(S.ctx, S.d_interp);
     if (S.mv_dev) tm_free(S.ctx, S.mv_dev);
+    if (S.mv_host) tm_host_free(S.ctx, S.mv_host);
     for (int f = 0; f < S.envsol_nf; f++)
       if (S.d_envsol[f]) tm_free(S.ctx, S.d_envsol[f]);
     if (S.d_envsrc) tm_free(S.ctx, S.d_envsrc);
@@ -167,29 +176,35 @@ void tm_compat_shutdown(void) {
   S.d_envsrc = NULL;
   S.envsol_nf = 0;
   S.envsol_nodes = 0;
-  free(S.mv_host);
   S.mv_host = S.mv_dev = NULL;
   S.mv_cap = 0;
   S.d_prod = S.d_field = S.d_grad = S.d_interp = NULL;
   S.grid_cap = S.interp_cap = 0;
   S.ctx = NULL;
   S.built_for = NULL;
-  free_host();
 }
 
+/* the per-cell staging arrays, in one page-locked block (the uploads and
+ * downloads through it run at the link rate) */
 static void ensure_host(systeminfo_t si, int64_t n) {
   if (n <= S.h_cap) return;
-  free_host();
-  S.h_x = malloc(sizeof(double) * n);
-  S.h_y = malloc(sizeof(double) * n);
-  S.h_z = malloc(sizeof(double) * n);
-  S.h_size = malloc(sizeof(double) * n);
-  S.h_out = malloc(sizeof(double) * 6 * n);
-  S.h_young = malloc(sizeof(float) * n);
-  S.h_ctype = malloc(sizeof(int32_t) * n);
-  S.h_phase = malloc(sizeof(int32_t) * n);
-  if (!S.h_x || !S.h_y || !S.h_z || !S.h_size || !S.h_out || !S.h_young || !S.h_ctype || !S.h_phase)
-    FAIL(si, "cannot allocate host staging");
+  if (S.h_block) tm_host_free(S.ctx, S.h_block);
+  S.h_block = NULL;
+  S.h_cap = 0;
+  const size_t a = ((size_t)n * sizeof(double) + 255) & ~(size_t)255; /* one array of doubles, aligned */
+  const size_t b = ((size_t)n * sizeof(int32_t) + 255) & ~(size_t)255;
+  void *p = NULL;
+  check(si, tm_host_alloc(S.ctx, &p, 10 * a + 3 * b), "host staging");
+  char *q = (char *)p;
+  S.h_x = (double *)q;
+  S.h_y = (double *)(q + a);
+  S.h_z = (double *)(q + 2 * a);
+  S.h_size = (double *)(q + 3 * a);
+  S.h_out = (double *)(q + 4 * a); /* 6 n doubles */
+  S.h_young = (float *)(q + 10 * a);
+  S.h_ctype = (int32_t *)(q + 10 * a + b);
+  S.h_phase = (int32_t *)(q + 10 * a + 2 * b);
+  S.h_block = p;
   S.h_cap = n;
 }
 
@@ -210,7 +225,8 @@ void octbuild(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype) {
   }
   tm_ctx *c = ctx_for(si);
   ensure_host(si, n);
-  int maxt = 0;
+  int maxt = 0, neg = 0;
+#pragma omp parallel for schedule(static) reduction(max : maxt) reduction(| : neg)
   for (int64_t i = 0; i < n; i++) {
     const celldata_t *cd = &ci->cells[i];
     S.h_x[i] = cd->x;
@@ -221,8 +237,9 @@ void octbuild(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype) {
     S.h_ctype[i] = cd->ctype;
     S.h_phase[i] = cd->phase;
     if (cd->ctype > maxt) maxt = cd->ctype;
-    if (cd->ctype < 0) FAIL(si, "negative cell type");
+    neg |= cd->ctype < 0;
   }
+  if (neg) FAIL(si, "negative cell type");
   int ntypes = maxt + 1;
   if (ntypes > 16) FAIL(si, "at most 16 cell types");
   tm_celltype t[16];
@@ -247,14 +264,34 @@ void octbuild(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype) {
   S.n = n;
   S.have_forces = 0;
   S.deposited = 0;
+  const octnode_t *prev = ci->octree;
   ci->octree = NULL;
   ci->octsize = 0;
   ci->octmaxsize = 0;
   if (S.host_octree) {
     int64_t cnt = 0;
     check(si, tm_get_octree(c, 0, NULL, &cnt), "tm_get_octree");
-    octnode_t *o = (octnode_t *)malloc(sizeof(octnode_t) * (size_t)cnt);
-    if (!o) FAIL(si, "cannot allocate cellsinfo->octree");
+    /* the kept page-locked buffer when it is free (or still this caller's,
+     * from an octbuild without octfree), else a plain allocation */
+    if (S.oct_out && prev == S.oct_buf) S.oct_out = 0;
+    octnode_t *o = NULL;
+    if (!S.oct_out) {
+      if (cnt > S.oct_cap) {
+        if (S.oct_buf) tm_host_free(c, S.oct_buf);
+        S.oct_buf = NULL;
+        S.oct_cap = 0;
+        void *p = NULL;
+        const int64_t want = cnt + cnt / 8;
+        check(si, tm_host_alloc(c, &p, sizeof(octnode_t) * (size_t)want), "cellsinfo->octree");
+        S.oct_buf = (octnode_t *)p;
+        S.oct_cap = want;
+      }
+      o = S.oct_buf;
+      S.oct_out = 1;
+    } else {
+      o = (octnode_t *)malloc(sizeof(octnode_t) * (size_t)cnt);
+      if (!o) FAIL(si, "cannot allocate cellsinfo->octree");
+    }
     check(si, tm_get_octree(c, 0, o, &cnt), "tm_get_octree");
     ci->octree = o;
     ci->octsize = cnt;
@@ -264,7 +301,8 @@ void octbuild(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype) {
 
 void octfree(cellsinfo_t *ci) {
   if (ci->localcount.n == 0) return; /* src/octree.c:323 */
-  free(ci->octree);
+  if (ci->octree && ci->octree == S.oct_buf) S.oct_out = 0; /* kept for the next octbuild */
+  else free(ci->octree);
   ci->octree = NULL;
   if (S.built_for == ci) S.built_for = NULL;
 }
@@ -375,6 +413,7 @@ void computepotential(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype, ce
   run_pairs(si, ci);
   const int64_t n = S.n;
   const double *o = S.h_out;
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; i++) {
     ci->cells[i].density = o[i];
     ci->cells[i].v = o[n + i];
@@ -392,6 +431,7 @@ void computegradient(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltype, cel
   if (!S.have_forces) run_pairs(si, ci);
   const int64_t n = S.n;
   const double *o = S.h_out;
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; i++) {
     ci->forces[i].x = o[3 * n + i];
     ci->forces[i].y = o[4 * n + i];
@@ -426,12 +466,11 @@ void computeremotegradient(systeminfo_t si, cellsinfo_t *ci, celltype_t *celltyp
 
 static void ensure_mv(systeminfo_t si, tm_ctx *c, int64_t n) {
   if (n <= S.mv_cap) return;
-  free(S.mv_host);
+  if (S.mv_host) tm_host_free(c, S.mv_host);
   if (S.mv_dev) tm_free(c, S.mv_dev);
-  S.mv_dev = NULL;
+  S.mv_host = S.mv_dev = NULL;
   S.mv_cap = 0;
-  S.mv_host = malloc((size_t)MV_BYTES * (size_t)n);
-  if (!S.mv_host) FAIL(si, "cannot allocate host staging");
+  check(si, tm_host_alloc(c, &S.mv_host, (size_t)MV_BYTES * (size_t)n), "host staging");
   check(si, tm_malloc(c, &S.mv_dev, (size_t)MV_BYTES * (size_t)n), "cell staging");
   S.mv_cap = n;
 }
@@ -721,31 +760,41 @@ void patches_alloc(systeminfo_t si, settings_t st, patches_t *pt, cellsinfo_t *c
   if (!pt->data || !pt->intersect || !pt->lowercorner || !pt->uppercorner || !pt->lowercornerR ||
       !pt->uppercornerR || !pt->size)
     FAIL(si, "cannot allocate patches");
-  int643dv_t lo = {INT64_MAX, INT64_MAX, ci->dimension == 3 ? INT64_MAX : 0};
-  int643dv_t hi = {INT64_MIN, INT64_MIN, ci->dimension == 3 ? INT64_MIN : 0};
+  const int d3 = ci->dimension == 3;
+  int64_t lox = INT64_MAX, loy = INT64_MAX, loz = d3 ? INT64_MAX : 0;
+  int64_t hix = INT64_MIN, hiy = INT64_MIN, hiz = d3 ? INT64_MIN : 0;
+  int any = 0;
   const int643dv_t L = grid->loweridx[0], U = grid->upperidx[0];
-  for (uint64_t c = 0; c < ci->localcount.n; c++) {
+  const double gx = grid->lowercorner.x, gy = grid->lowercorner.y, gz = grid->lowercorner.z;
+  const double res = grid->resolution;
+  const int64_t n = (int64_t)ci->localcount.n;
+  /* minima, maxima and "any": the same result in any order */
+#pragma omp parallel for schedule(static) reduction(min : lox, loy, loz) reduction(max : hix, hiy, hiz) \
+    reduction(| : any)
+  for (int64_t c = 0; c < n; c++) {
     int643dv_t k;
-    k.x = (int64_t)((ci->cells[c].x - grid->lowercorner.x) / grid->resolution);
-    k.y = (int64_t)((ci->cells[c].y - grid->lowercorner.y) / grid->resolution);
-    k.z = (int64_t)((ci->cells[c].z - grid->lowercorner.z) / grid->resolution);
+    k.x = (int64_t)((ci->cells[c].x - gx) / res);
+    k.y = (int64_t)((ci->cells[c].y - gy) / res);
+    k.z = (int64_t)((ci->cells[c].z - gz) / res);
     for (int ax = 0; ax < 2; ax++)
       for (int ay = 0; ay < 2; ay++)
         for (int az = 0; az < 2; az++) {
           if (!(k.x + ax >= L.x && k.y + ay >= L.y && k.z + az >= L.z && k.x + ax <= U.x && k.y + ay <= U.y &&
                 k.z + az <= U.z))
             continue;
-          if (k.x + ax < lo.x) lo.x = k.x + ax;
-          if (k.y + ay < lo.y) lo.y = k.y + ay;
-          if (k.x + ax > hi.x) hi.x = k.x + ax;
-          if (k.y + ay > hi.y) hi.y = k.y + ay;
-          if (ci->dimension == 3) {
-            if (k.z + az < lo.z) lo.z = k.z + az;
-            if (k.z + az > hi.z) hi.z = k.z + az;
+          if (k.x + ax < lox) lox = k.x + ax;
+          if (k.y + ay < loy) loy = k.y + ay;
+          if (k.x + ax > hix) hix = k.x + ax;
+          if (k.y + ay > hiy) hiy = k.y + ay;
+          if (d3) {
+            if (k.z + az < loz) loz = k.z + az;
+            if (k.z + az > hiz) hiz = k.z + az;
           }
-          pt->intersect[0] = 1;
+          any = 1;
         }
   }
+  const int643dv_t lo = {lox, loy, loz}, hi = {hix, hiy, hiz};
+  if (any) pt->intersect[0] = 1;
   pt->lowercorner[0] = lo;
   pt->uppercorner[0] = hi;
   if (pt->intersect[0]) {

@@ -344,6 +344,8 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_chunk_start);
   free_ptr(c->d_chunk_cid);
   free_ptr(c->d_cbox);
+  free_ptr(c->d_octcnt);
+  free_ptr(c->d_octout);
   free_ptr(c->d_tile_idx);
   free_ptr(c->d_tile_cnt);
   free_ptr(c->d_tile_cell);
@@ -686,6 +688,23 @@ tm_status tm_free(tm_ctx *c, void *ptr) {
   if (!c) return TM_ERR_INVALID;
   TM_HIP(c, hipStreamSynchronize(c->stream));
   if (ptr) TM_HIP(c, hipFree(ptr));
+  return TM_OK;
+}
+
+tm_status tm_host_alloc(tm_ctx *c, void **ptr, size_t bytes) {
+  if (!c || !ptr) return TM_ERR_INVALID;
+  TM_HIP(c, hipSetDevice(c->device));
+  if (hipHostMalloc(ptr, bytes, hipHostMallocDefault) != hipSuccess) {
+    *ptr = nullptr;
+    return tm_fail(c, TM_ERR_NOMEM, "hipHostMalloc failed");
+  }
+  return TM_OK;
+}
+
+tm_status tm_host_free(tm_ctx *c, void *ptr) {
+  if (!c) return TM_ERR_INVALID;
+  TM_HIP(c, hipStreamSynchronize(c->stream));
+  if (ptr) TM_HIP(c, hipHostFree(ptr));
   return TM_OK;
 }
 

@@ -126,6 +126,12 @@ struct tm_ctx {
   int32_t *d_delta = nullptr;
   int32_t *d_split = nullptr;   // split position of each internal node
   int32_t *d_parent = nullptr;  // parent of every node (2n-1), -1 at roots
+  // the emitted octree (tm_get_octree): chain counts and the node records,
+  // kept between calls
+  uint32_t *d_octcnt = nullptr;
+  int64_t cap_octcnt = 0;
+  char *d_octout = nullptr;
+  int64_t cap_octout = 0;  // bytes
   int64_t cap_tree_aux = 0;
 
   // neighbours: slot k of cell p at nbr[tm_slot(p, k, nbr_alloc)]

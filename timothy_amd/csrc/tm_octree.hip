@@ -185,8 +185,8 @@ tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count) {
   if (root < 0 || root > n - 2) return tm_fail(c, TM_ERR_STATE, "colony has no internal root");
   k_fill_i32<<<tm_blocks(2 * n - 1, 256), 256, 0, c->stream>>>(c->d_parent, 2 * n - 1, -1);
   k_parents<<<tm_blocks(n - 1, 256), 256, 0, c->stream>>>(c->d_range, c->d_split, n, c->d_parent);
-  uint32_t *cnt = nullptr;
-  TM_HIP(c, hipMalloc((void **)&cnt, sizeof(uint32_t) * (size_t)(nc + 1)));
+  TM_TRY(tm_ensure(c, &c->d_octcnt, c->cap_octcnt, nc + 1, "octree chain counts"));
+  uint32_t *cnt = c->d_octcnt;
   k_chain_counts<<<tm_blocks(nc, 256), 256, 0, c->stream>>>(c->d_range, c->d_delta, c->d_parent, n, s, e, root, cnt);
   tm_status st = tm_scan_u32(c, cnt, cnt, nc + 1);
   uint32_t ninternal = 0;
@@ -199,11 +199,8 @@ tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count) {
   if (st == TM_OK) {
     *count = total;
     if (nodes) {
-      OctNode *d_out = nullptr;
-      if (hipMalloc((void **)&d_out, sizeof(OctNode) * (size_t)total) != hipSuccess) {
-        (void)hipFree(cnt);
-        return tm_fail(c, TM_ERR_NOMEM, "hipMalloc failed for octree nodes");
-      }
+      TM_TRY(tm_ensure(c, &c->d_octout, c->cap_octout, (int64_t)sizeof(OctNode) * total, "octree nodes"));
+      OctNode *d_out = reinterpret_cast<OctNode *>(c->d_octout);
       k_emit_internal<<<tm_blocks(nc, 256), 256, 0, c->stream>>>(c->d_range, c->d_delta, c->d_parent, c->d_code, cnt,
                                                                   n, s, e, root, d_out);
       k_emit_leaves<<<tm_blocks(nc, 256), 256, 0, c->stream>>>(c->d_delta, c->d_parent, c->d_code, c->d_span,
@@ -213,10 +210,8 @@ tm_status tm_run_octree(tm_ctx *c, int colony, void *nodes, int64_t *count) {
       if (he == hipSuccess)
         he = hipMemcpyAsync(nodes, d_out, sizeof(OctNode) * (size_t)total, hipMemcpyDeviceToHost, c->stream);
       if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
-      (void)hipFree(d_out);
       if (he != hipSuccess) st = tm_fail(c, TM_ERR_HIP, hipGetErrorString(he));
     }
   }
-  (void)hipFree(cnt);
   return st;
 }
