@@ -210,6 +210,10 @@ tm_status tm_free(tm_ctx *ctx, void *ptr);
 tm_status tm_memcpy_htod(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 tm_status tm_memcpy_dtoh(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 tm_status tm_memset(tm_ctx *ctx, void *dst, int value, size_t bytes);
+/* the same copies without waiting: they run on the context's stream, in
+ * order with its kernels; tm_sync waits for them */
+tm_status tm_memcpy_htod_async(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
+tm_status tm_memcpy_dtoh_async(tm_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* page-locked host memory: copies between it and the device run at the
  * full link rate (the compat layer stages cells and the emitted octree in
  * it).  tm_host_free takes only pointers tm_host_alloc returned. */

@@ -126,6 +126,8 @@ def _load():
         "tm_memcpy_dtoh": (_i32, [_vp, _vp, _vp, C.c_size_t]),
         "tm_memset": (_i32, [_vp, _vp, C.c_int, C.c_size_t]),
         "tm_host_alloc": (_i32, [_vp, C.POINTER(_vp), C.c_size_t]),
+        "tm_memcpy_htod_async": (_i32, [_vp, _vp, _vp, C.c_size_t]),
+        "tm_memcpy_dtoh_async": (_i32, [_vp, _vp, _vp, C.c_size_t]),
         "tm_host_free": (_i32, [_vp, _vp]),
         "tm_memcpy_dtod": (_i32, [_vp, _vp, _vp, C.c_size_t]),
         "tm_time_stage": (_i32, [_vp, C.c_int, C.c_int, _i32, _i32, C.POINTER(C.c_double)]),

@@ -54,6 +54,9 @@ static struct {
   octnode_t *oct_buf;
   int64_t oct_cap;
   int oct_out;
+  /* two page-locked chunks for copies from and to the caller's own
+   * (pageable) arrays */
+  void *stage[2];
   /* grid transfer buffers */
   double *d_prod, *d_field, *d_grad, *d_interp;
   int64_t grid_cap, interp_cap;
@@ -147,6 +150,10 @@ static void free_host(void) {
   S.h_young = NULL;
   S.h_ctype = S.h_phase = NULL;
   S.h_cap = 0;
+  for (int b = 0; b < 2; b++) {
+    if (S.stage[b] && S.ctx) tm_host_free(S.ctx, S.stage[b]);
+    S.stage[b] = NULL;
+  }
   /* the octree buffer, unless a caller still holds it (octfree then takes it
    * back; it lives on until the process ends) */
   if (S.oct_buf && S.ctx && !S.oct_out) {
@@ -182,6 +189,56 @@ void tm_compat_shutdown(void) {
   S.grid_cap = S.interp_cap = 0;
   S.ctx = NULL;
   S.built_for = NULL;
+}
+
+/* Copies between the caller's pageable arrays and the device, pipelined
+ * through two page-locked chunks: threads copy one chunk on the host while
+ * the previous one crosses the link (a pageable copy stages through the
+ * runtime's buffers with one host thread). */
+#define STAGE_BYTES ((size_t)32 << 20)
+
+static void ensure_stage(systeminfo_t si) {
+  for (int b = 0; b < 2; b++)
+    if (!S.stage[b]) check(si, tm_host_alloc(S.ctx, &S.stage[b], STAGE_BYTES), "copy staging");
+}
+
+static void par_memcpy(void *dst, const void *src, size_t bytes) {
+  const size_t blk = (size_t)1 << 20;
+  const int64_t nb = (int64_t)((bytes + blk - 1) / blk);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nb; i++) {
+    const size_t o = (size_t)i * blk, m = bytes - o < blk ? bytes - o : blk;
+    memcpy((char *)dst + o, (const char *)src + o, m);
+  }
+}
+
+static void copy_to_device(systeminfo_t si, void *dst, const void *src, size_t bytes, const char *what) {
+  ensure_stage(si);
+  for (size_t o = 0, i = 0; o < bytes; o += STAGE_BYTES, i++) {
+    const size_t m = bytes - o < STAGE_BYTES ? bytes - o : STAGE_BYTES;
+    void *buf = S.stage[i & 1];
+    par_memcpy(buf, (const char *)src + o, m); /* while chunk i - 1 crosses from the other buffer */
+    check(si, tm_sync(S.ctx), what);         /* chunk i - 1 is across: its buffer is free again */
+    check(si, tm_memcpy_htod_async(S.ctx, (char *)dst + o, buf, m), what);
+  }
+  check(si, tm_sync(S.ctx), what);
+}
+
+static void copy_to_host(systeminfo_t si, void *dst, const void *src, size_t bytes, const char *what) {
+  ensure_stage(si);
+  if (bytes == 0) return;
+  size_t m0 = bytes < STAGE_BYTES ? bytes : STAGE_BYTES;
+  check(si, tm_memcpy_dtoh_async(S.ctx, S.stage[0], src, m0), what);
+  for (size_t o = 0, i = 0; o < bytes; o += STAGE_BYTES, i++) {
+    const size_t m = bytes - o < STAGE_BYTES ? bytes - o : STAGE_BYTES;
+    check(si, tm_sync(S.ctx), what); /* chunk i is in its buffer */
+    const size_t o1 = o + STAGE_BYTES;
+    if (o1 < bytes) {
+      const size_t m1 = bytes - o1 < STAGE_BYTES ? bytes - o1 : STAGE_BYTES;
+      check(si, tm_memcpy_dtoh_async(S.ctx, S.stage[(i + 1) & 1], (const char *)src + o1, m1), what);
+    }
+    par_memcpy((char *)dst + o, S.stage[i & 1], m); /* while chunk i + 1 crosses */
+  }
 }
 
 /* the per-cell staging arrays, in one page-locked block (the uploads and
@@ -834,7 +891,7 @@ void patches_cells2envwait(systeminfo_t si, settings_t st, patches_t *pt, grid_t
   if (!S.deposited) FAIL(si, "patches_cells2envwait without patches_cells2envinit");
   const int64_t nodes = grid_nodes(grid);
   for (int f = 1; f < st.numberoffields; f++) memset((*env)[f].production, 0, sizeof(double) * nodes);
-  check(si, tm_memcpy_dtoh(S.ctx, (*env)[0].production, S.d_prod, sizeof(double) * nodes), "production");
+  copy_to_host(si, (*env)[0].production, S.d_prod, sizeof(double) * nodes, "production");
   (void)pt;
 }
 
@@ -870,9 +927,8 @@ void patches_env2cellsinit(systeminfo_t si, settings_t st, patches_t *pt, grid_t
     S.env_nodes = nodes;
   }
   for (int f = 0; f < nf; f++) {
-    check(si, tm_memcpy_htod(S.ctx, S.d_envf[f], (*env)[f].data, sizeof(double) * nodes), "field upload");
-    check(si, tm_memcpy_htod(S.ctx, S.d_envg[f], (*env)[f].gradient, sizeof(double) * 3 * nodes),
-          "gradient upload");
+    copy_to_device(si, S.d_envf[f], (*env)[f].data, sizeof(double) * nodes, "field upload");
+    copy_to_device(si, S.d_envg[f], (*env)[f].gradient, sizeof(double) * 3 * nodes, "gradient upload");
   }
   S.env_ready = 1;
   (void)pt;
@@ -903,7 +959,7 @@ void patches_env2cellswait(systeminfo_t si, settings_t st, patches_t *pt, cellsi
       FAIL(si, "cannot allocate cellenvdata[f]");
     if (n == 0) continue;
     check(si, tm_cic_interp(S.ctx, &g, S.d_envf[f], S.d_envg[f], S.d_interp), "CIC interpolation");
-    check(si, tm_memcpy_dtoh(S.ctx, (*ced)[f], S.d_interp, sizeof(cellenvdata_t) * n), "cellenvdata");
+    copy_to_host(si, (*ced)[f], S.d_interp, sizeof(cellenvdata_t) * n, "cellenvdata");
   }
   S.env_ready = 0;
   (void)pt;
@@ -917,9 +973,9 @@ void fieldgradient(systeminfo_t si, settings_t st, environment_t **env, grid_t *
   ensure_grid(si, nodes);
   tm_grid g = grid_of(grid, si.rank);
   for (int f = 0; f < st.numberoffields; f++) {
-    check(si, tm_memcpy_htod(S.ctx, S.d_field, (*env)[f].data, sizeof(double) * nodes), "field upload");
+    copy_to_device(si, S.d_field, (*env)[f].data, sizeof(double) * nodes, "field upload");
     check(si, tm_field_gradient(S.ctx, &g, S.d_field, S.d_grad), "field gradient");
-    check(si, tm_memcpy_dtoh(S.ctx, (*env)[f].gradient, S.d_grad, sizeof(double) * 3 * nodes), "gradient");
+    copy_to_host(si, (*env)[f].gradient, S.d_grad, sizeof(double) * 3 * nodes, "gradient");
   }
 }
 

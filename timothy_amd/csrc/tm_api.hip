@@ -708,6 +708,18 @@ tm_status tm_host_free(tm_ctx *c, void *ptr) {
   return TM_OK;
 }
 
+tm_status tm_memcpy_htod_async(tm_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c) return TM_ERR_INVALID;
+  TM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return TM_OK;
+}
+
+tm_status tm_memcpy_dtoh_async(tm_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!c) return TM_ERR_INVALID;
+  TM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return TM_OK;
+}
+
 tm_status tm_memcpy_htod(tm_ctx *c, void *dst, const void *src, size_t bytes) {
   if (!c) return TM_ERR_INVALID;
   TM_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
