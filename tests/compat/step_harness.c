@@ -125,6 +125,25 @@ int main(int argc, char **argv) {
   memset(&cc, 0, sizeof(cc));
   cellenvdata_t **ced = NULL;
 
+  /* STEP_HARNESS_PREBUILD=free|held: first build the octree of a moved
+   * copy of the cells (released with octfree before the step, or held
+   * until the end), so the step's octbuild reuses the layer's kept octree
+   * buffer or, while the copy holds it, gets a plain allocation */
+  const char *pre = getenv("STEP_HARNESS_PREBUILD");
+  cellsinfo_t ci2;
+  memset(&ci2, 0, sizeof(ci2));
+  if (pre && *pre) {
+    ci2 = ci;
+    ci2.cells = calloc(n, sizeof(celldata_t));
+    for (int64_t i = 0; i < n; i++) {
+      ci2.cells[i] = ci.cells[i];
+      ci2.cells[i].x = 0.9 * ci.cells[i].x + 1.0;
+      ci2.cells[i].y = 0.8 * ci.cells[i].y - 2.0;
+    }
+    octbuild(si, &ci2, ct);
+    if (!strcmp(pre, "free")) octfree(&ci2);
+  }
+
   /* src/main.c:70 then src/step.c:47-97 */
   octbuild(si, &ci, ct);
   patches_t patches;
@@ -211,6 +230,8 @@ int main(int argc, char **argv) {
   }
 
   octfree(&ci);
+  if (pre && !strcmp(pre, "held")) octfree(&ci2);
+  free(ci2.cells);
   for (int64_t k = 0; k < nf; k++) free(ced[k]);
   free(ced);
   tm_compat_shutdown();

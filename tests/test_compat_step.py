@@ -65,7 +65,7 @@ def _read_output(path, n, G, nf, nwalk):
                 woff=woff, wc=wc)
 
 
-def _run(tmp_path, col, precision, move=None):
+def _run(tmp_path, col, precision, move=None, prebuild=""):
     if not os.path.exists(B.HARNESS):
         pytest.fail(f"{B.HARNESS} is not built (run the package build first)")
     grid = col["grid"]
@@ -75,8 +75,8 @@ def _run(tmp_path, col, precision, move=None):
     grads = [np.concatenate([g, 0.5 * g, -g]) for g in grads]
     inp, out = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     _write_input(inp, col, fields, grads)
-    env = dict(os.environ, TM_PRECISION=precision)
-    print(f"step_harness n={col['n']} G={G} precision={precision}", flush=True)
+    env = dict(os.environ, TM_PRECISION=precision, STEP_HARNESS_PREBUILD=prebuild)
+    print(f"step_harness n={col['n']} G={G} precision={precision} prebuild={prebuild!r}", flush=True)
     args = [B.HARNESS, inp, out]
     if move is not None:
         args += [str(move[0]), repr(float(move[1])), str(tmp_path / "move.bin")]
@@ -99,13 +99,17 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("name,make", CASES)
-def test_step_compute_order_fp64(oracle_mod, tmp_path, name, make):
+# prebuild: the octree of a moved copy of the cells is built first and
+# released ("free": the step reuses the layer's kept octree buffer) or held
+# ("held": the step's octree gets a plain allocation)
+@pytest.mark.parametrize("name,make,prebuild", [c + ("",) for c in CASES] + [CASES[0] + ("free",),
+                                                                           CASES[0] + ("held",)])
+def test_step_compute_order_fp64(oracle_mod, tmp_path, name, make, prebuild):
     O = oracle_mod
     col = make()
     if "grid" not in col:
         col["grid"] = gen.bbox_grid(col, 64)
-    got, fields, grads = _run(tmp_path, col, "fp64")
+    got, fields, grads = _run(tmp_path, col, "fp64", prebuild=prebuild)
     oc = O.Colony.from_dict(col)
     tree = O.Octree(oc, col["types"])
     # octbuild: affine map and node count
