@@ -352,8 +352,7 @@ void tm_destroy(tm_ctx *c) {
   free_ptr(c->d_nbr16);
   free_ptr(c->d_cg);
   free_ptr(c->d_dkr);
-  free_ptr(c->d_drec);
-  free_ptr(c->d_dloc);
+  free_ptr(c->d_didx);
   free_ptr(c->d_drange);
   free_ptr(c->d_vtk);
   if (c->h_flags) (void)hipHostFree(c->h_flags);

@@ -200,41 +200,19 @@ __global__ __launch_bounds__(256) void k_dep_count(const double4 *__restrict__ p
   }
 }
 
-// Writes each keyed cell's record at its place in key order: (d_x, d_y, d_z,
-// weight), and its corner index relative to its tile's first node plus one
-// (0 .. DT per axis, 5 bits each), so that the tile pass needs no division.
-// The rank-0 cell of each key clears the key's count (the scan has read it),
-// so the counts are zero after the deposit.
-__global__ __launch_bounds__(256) void k_dep_place(const double4 *__restrict__ pos,
-                                                   const int32_t *__restrict__ phase_s,
-                                                   const uint64_t *__restrict__ kr, int64_t n, DepGrid g,
-                                                   int weight_mode, uint32_t nkeys, const uint32_t *__restrict__ offs,
-                                                   uint32_t *__restrict__ cnt, double4 *__restrict__ rec,
-                                                   uint32_t *__restrict__ loc) {
+// Writes each keyed cell's index at its place in key order.  The rank-0
+// cell of each key clears the key's count (the scan has read it), so the
+// counts are zero after the deposit.
+__global__ __launch_bounds__(256) void k_dep_place(const uint64_t *__restrict__ kr, int64_t n, uint32_t nkeys,
+                                                   const uint32_t *__restrict__ offs, uint32_t *__restrict__ cnt,
+                                                   uint32_t *__restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint64_t e = kr[i];
   const uint32_t k = (uint32_t)e, r = (uint32_t)(e >> 32);
   if (k >= nkeys) return;
-  const double4 P = pos[i];
-  const int ph = weight_mode != 1 ? phase_s[i] : 1;
-  const uint32_t slot = offs[k] + r;
   if (r == 0) cnt[k] = 0u;
-  const uint32_t tile = k >> 3;
-  const int tl[3] = {(int)(tile / ((uint32_t)g.nt[2] * (uint32_t)g.nt[1])),
-                     (int)((tile / (uint32_t)g.nt[2]) % (uint32_t)g.nt[1]), (int)(tile % (uint32_t)g.nt[2])};
-  const double pc[3] = {P.x, P.y, P.z};
-  double d[3];
-  uint32_t packed = 0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    int ci;
-    double w0;
-    (void)dep_axis(g, a, pc[a], ci, w0, d[a]);  // a keyed cell reaches the grid
-    packed |= (uint32_t)(ci - (g.lo[a] + DT * tl[a]) + 1) << (5 * a);
-  }
-  rec[slot] = make_double4(d[0], d[1], d[2], dep_wt(P, ph, weight_mode));
-  loc[slot] = packed;
+  idx[offs[k] + r] = (uint32_t)i;
 }
 
 // Exclusive scan of n words by one workgroup, out[n] the total, for the
@@ -386,9 +364,13 @@ constexpr int DEP_U = 4;  // cells per thread and round in k_dep_tile
 
 // One workgroup per tile, in launch-slot order.  The tile's runs are laid
 // end to end as one index space that every thread strides through, DEP_U
-// cells per round with all loads issued first.
-__global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ rec,
-                                                  const uint32_t *__restrict__ loc,
+// cells per round with all loads issued first.  A cell's corner index and
+// weights are formed here from its position, as the key was (its corner
+// index relative to this tile's first node is -1 for a cell of a lower
+// neighbour's last plane).
+__global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ pos,
+                                                  const int32_t *__restrict__ phase_s, int weight_mode,
+                                                  const uint32_t *__restrict__ idx,
                                                   const uint32_t *__restrict__ slots, DepGrid g, int64_t ls1,
                                                   int64_t ls2, double *__restrict__ prod) {
   __shared__ double acc[DT * DT * DT];
@@ -402,34 +384,48 @@ __global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ 
   const int T[3] = {g.lo[0] + DT * tx, g.lo[1] + DT * ty, g.lo[2] + DT * tz};
   const int vhi[3] = {min(g.hi[0] - T[0], DT - 1), min(g.hi[1] - T[1], DT - 1), min(g.hi[2] - T[2], DT - 1)};
   const uint32_t total = w[2 * DEP_R];
-  for (uint32_t v0 = 0; v0 < total; v0 += DEP_U * DEP_T) {
-    uint32_t p[DEP_U], dd[DEP_U];
+  // the cells of round v0 (the next round's indices are read while this
+  // round's positions are)
+  auto cells_of = [&](uint32_t v0, uint32_t c[DEP_U]) {
 #pragma unroll
     for (int u = 0; u < DEP_U; ++u) {
       const uint32_t v = v0 + DEP_T * u + threadIdx.x;
-      p[u] = UINT32_MAX;
-      dd[u] = 0;
+      c[u] = UINT32_MAX;
       if (v < total) {
         int r = 0;
         while (w[1 + DEP_R + r] <= v) ++r;  // ends at r < DEP_R since v < total
-        p[u] = w[1 + r] + (v - (r == 0 ? 0u : w[DEP_R + r]));
-        dd[u] = dep_runs[r] & 7u;
+        c[u] = idx[w[1 + r] + (v - (r == 0 ? 0u : w[DEP_R + r]))];
       }
     }
-    double4 R[DEP_U];
-    uint32_t L[DEP_U];
+  };
+  uint32_t cn[DEP_U];
+  cells_of(0, cn);
+  for (uint32_t v0 = 0; v0 < total; v0 += DEP_U * DEP_T) {
+    uint32_t c[DEP_U];
+#pragma unroll
+    for (int u = 0; u < DEP_U; ++u) c[u] = cn[u];
+    double4 P[DEP_U];
+    int ph[DEP_U];
 #pragma unroll
     for (int u = 0; u < DEP_U; ++u) {
-      R[u] = p[u] != UINT32_MAX ? rec[p[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
-      L[u] = p[u] != UINT32_MAX ? loc[p[u]] : 0u;
+      P[u] = c[u] != UINT32_MAX ? pos[c[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
+      ph[u] = c[u] != UINT32_MAX && weight_mode != 1 ? phase_s[c[u]] : 1;
     }
+    cells_of(v0 + DEP_U * DEP_T, cn);
 #pragma unroll
     for (int u = 0; u < DEP_U; ++u) {
-      if (p[u] == UINT32_MAX) continue;
+      if (c[u] == UINT32_MAX) continue;
+      const double pc[3] = {P[u].x, P[u].y, P[u].z};
+      double d[3];
       int l[3];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) l[a] = (int)((L[u] >> (5 * a)) & 31u) - 1 - (int)((dd[u] >> a) & 1u) * DT;
-      dep_cell(R[u], l, vhi, acc);
+      for (int a = 0; a < 3; ++a) {
+        int ci;
+        double w0;
+        (void)dep_axis(g, a, pc[a], ci, w0, d[a]);  // a keyed cell reaches the grid
+        l[a] = ci - T[a];
+      }
+      dep_cell(make_double4(d[0], d[1], d[2], dep_wt(P[u], ph[u], weight_mode)), l, vhi, acc);
     }
   }
   __syncthreads();
@@ -534,8 +530,7 @@ tm_status tm_run_deposit(tm_ctx *c, const tm_grid *grid, int weight_mode, double
   c->dep_zero = 0;
   if (n > 0) {
     TM_TRY(tm_ensure(c, &c->d_dkr, c->cap_dkr, n, "deposit keys"));
-    TM_TRY(tm_ensure(c, &c->d_drec, c->cap_drec, n, "deposit records"));
-    TM_TRY(tm_ensure(c, &c->d_dloc, c->cap_dloc, n, "deposit corner indices"));
+    TM_TRY(tm_ensure(c, &c->d_didx, c->cap_didx, n, "deposit cell order"));
     k_dep_count<<<tm_blocks(n, 256 * DEP_CU), 256, 0, c->stream>>>(c->d_pos, c->d_phase_s, n, g, weight_mode,
                                                           (uint32_t)nkeys, c->d_dkr, cnt, ctr);
   }
@@ -545,12 +540,11 @@ tm_status tm_run_deposit(tm_ctx *c, const tm_grid *grid, int weight_mode, double
     TM_TRY(tm_scan_u32(c, cnt, offs, nkeys + 1));
   }
   if (n > 0)
-    k_dep_place<<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_pos, c->d_phase_s, c->d_dkr, n, g, weight_mode,
-                                                          (uint32_t)nkeys, offs, cnt, c->d_drec, c->d_dloc);
+    k_dep_place<<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_dkr, n, (uint32_t)nkeys, offs, cnt, c->d_didx);
   k_dep_order<<<tm_blocks(ntiles, 256), 256, 0, c->stream>>>(offs, g, (uint32_t)ntiles, ctr, slots);
-  // with no cells every offset is 0 and no record is read
-  k_dep_tile<<<(unsigned)ntiles, DEP_T, 0, c->stream>>>(c->d_drec, c->d_dloc, slots, g, grid->localsize[1],
-                                                        grid->localsize[2], prod);
+  // with no cells every offset is 0 and no cell is read
+  k_dep_tile<<<(unsigned)ntiles, DEP_T, 0, c->stream>>>(c->d_pos, c->d_phase_s, weight_mode, c->d_didx, slots, g,
+                                                        grid->localsize[1], grid->localsize[2], prod);
   TM_HIP(c, hipGetLastError());
   c->dep_zero = nkeys + 1;
   return TM_OK;

@@ -186,16 +186,15 @@ struct tm_ctx {
   int64_t cap_vtk = 0;
   tm_precision last_prec = TM_FP64;
 
-  // deposit by grid tile: each cell's (tile, layer) key and rank; in key order each
-  // cell's record (corner offsets d, weight) and packed local corner index;
-  // per key a count and an offset, the tiles' launch slots.  The counts
+  // deposit by grid tile: each cell's (tile, layer) key and rank; the cells'
+  // indices in key order; per key a count and an offset, the tiles' launch
+  // slots.  The counts
   // are zero after a deposit: dep_zero words at the start of d_drange are
   // known to be zero.
   uint64_t *d_dkr = nullptr;  // key | rank within the key << 32
-  double4 *d_drec = nullptr;
-  uint32_t *d_dloc = nullptr;
+  uint32_t *d_didx = nullptr;  // the keyed cells in key order
   uint32_t *d_drange = nullptr;
-  int64_t cap_dkr = 0, cap_drec = 0, cap_dloc = 0, cap_drange = 0;
+  int64_t cap_dkr = 0, cap_didx = 0, cap_drange = 0;
   int64_t dep_zero = 0;
 
   // last deposit, for timing
