@@ -422,6 +422,20 @@ def test_cic_deposit_grid_sequence(tm, oracle_mod, ctx):
         assert np.array_equal(got, ref) and (mode == 1 or not ref.any()), mode
         prod.free()
     assert np.array_equal(np.asarray(col["phase"]), ph)
+    # mixed phases: G0 cells (phase 0) weigh 0.75 under mode 0, necrotic
+    # cells (phase 5) deposit nothing, the others weigh 1
+    mixed = dict(col, phase=np.random.default_rng(11).integers(0, 6, col["n"]).astype(np.int32))
+    ctx.upload_colony(mixed)
+    ctx.build()
+    for grid, mode in ((g0, 0), (g0, 1), (grids[2][0], 0)):
+        prod = tm.DeviceArray(ctx, int(np.prod(grid["localsize"])), np.float64)
+        ctx.cic_deposit(grid, prod, mode)
+        got = prod.get()
+        ref = O.cic_deposit(O.Colony.from_dict(mixed), grid, mode)
+        assert np.array_equal(got != 0, ref != 0), mode
+        assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref)), mode
+        say(f"  mixed phases mode {mode}: {int((ref != 0).sum())} nodes")
+        prod.free()
     ctx.upload_colony(col)
     ctx.build()
     # node indices outside (-2^29, 2^29) are refused before any launch

@@ -144,23 +144,34 @@ __device__ __forceinline__ uint32_t dep_key(const double4 &P, bool deposits, con
 }
 
 // Lanes of the wave holding the same key are served by one atomic on
-// ctr[key] (consecutive cells of the Morton order share keys); returns this
-// lane's rank among the key's cells: ctr[key]'s old value plus its rank
-// among those lanes.
-__device__ __forceinline__ uint32_t wave_rank(uint32_t key, bool active, uint32_t *__restrict__ ctr) {
-  uint32_t mine = 0;
+// ctr[key] (consecutive cells of the Morton order share keys); a lane's rank
+// among the key's cells is ctr[key]'s old value plus its rank among those
+// lanes.  wave_rank_issue has each key's first lane issue its atomic and
+// keep the old value; wave_rank_read, after every issue, has each lane read
+// its key's value from that lane, so the wave waits once for all of them.
+struct WaveRank {
+  uint32_t held, from, below;
+};
+
+__device__ __forceinline__ WaveRank wave_rank_issue(uint32_t key, bool active, uint32_t *__restrict__ ctr) {
+  WaveRank w{0u, 0u, 0u};
   uint64_t left = __ballot(active);
   while (left) {
     const int lead = __builtin_ctzll(left);
     const uint32_t lk = (uint32_t)__shfl((int)key, lead, 64);
     const uint64_t same = __ballot(active && key == lk);
-    uint32_t base = 0;
-    if (__lane_id() == (unsigned)lead) base = atomicAdd(ctr + lk, (uint32_t)__builtin_popcountll(same));
-    base = (uint32_t)__shfl((int)base, lead, 64);
-    if (active && key == lk) mine = base + (uint32_t)__builtin_popcountll(same & tm_lanemask_lt());
+    if (__lane_id() == (unsigned)lead) w.held = atomicAdd(ctr + lk, (uint32_t)__builtin_popcountll(same));
+    if (active && key == lk) {
+      w.from = (uint32_t)lead;
+      w.below = (uint32_t)__builtin_popcountll(same & tm_lanemask_lt());
+    }
     left &= ~same;
   }
-  return mine;
+  return w;
+}
+
+__device__ __forceinline__ uint32_t wave_rank_read(const WaveRank &w) {
+  return (uint32_t)__shfl((int)w.held, (int)w.from, 64) + w.below;
 }
 
 __global__ void k_fill_u32(uint32_t *p, int64_t n) {
@@ -175,8 +186,9 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n) {
 // first.
 constexpr int DEP_CU = 2;
 
-// kr[i] = key | rank << 32 (key nkeys: deposits nothing).  Block 0 also
-// clears the two launch-order counters ctr.
+// kr[i] = key | rank << 32 | (G0 cell under weight mode 0) << 63 (key
+// nkeys: deposits nothing).  Block 0 also clears the two launch-order
+// counters ctr.
 __global__ __launch_bounds__(256) void k_dep_count(const double4 *__restrict__ pos,
                                                    const int32_t *__restrict__ phase_s, int64_t n, DepGrid g,
                                                    int weight_mode, uint32_t nkeys, uint64_t *__restrict__ kr,
@@ -191,16 +203,24 @@ __global__ __launch_bounds__(256) void k_dep_count(const double4 *__restrict__ p
     P[u] = i < n ? pos[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     ph[u] = i < n && weight_mode != 1 ? phase_s[i] : 1;
   }
+  uint32_t k[DEP_CU];
+  WaveRank w[DEP_CU];
 #pragma unroll
   for (int u = 0; u < DEP_CU; ++u) {
     const int64_t i = i0 + 256 * u;
-    const uint32_t k = i < n ? dep_key(P[u], ph[u] != 5, g, nkeys) : nkeys;
-    const uint32_t r = wave_rank(k, k < nkeys, cnt);
-    if (i < n) kr[i] = (uint64_t)k | (uint64_t)r << 32;
+    k[u] = i < n ? dep_key(P[u], ph[u] != 5, g, nkeys) : nkeys;
+    w[u] = wave_rank_issue(k[u], k[u] < nkeys, cnt);
+  }
+#pragma unroll
+  for (int u = 0; u < DEP_CU; ++u) {
+    const int64_t i = i0 + 256 * u;
+    const uint32_t r = wave_rank_read(w[u]);
+    if (i < n) kr[i] = (uint64_t)k[u] | (uint64_t)r << 32 | (uint64_t)(weight_mode != 1 && ph[u] == 0) << 63;
   }
 }
 
-// Writes each keyed cell's index at its place in key order.  The rank-0
+// Writes each keyed cell's index at its place in key order, with bit 31
+// set for a G0 cell under weight mode 0 (its weight is 0.75).  The rank-0
 // cell of each key clears the key's count (the scan has read it), so the
 // counts are zero after the deposit.
 __global__ __launch_bounds__(256) void k_dep_place(const uint64_t *__restrict__ kr, int64_t n, uint32_t nkeys,
@@ -209,10 +229,10 @@ __global__ __launch_bounds__(256) void k_dep_place(const uint64_t *__restrict__ 
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint64_t e = kr[i];
-  const uint32_t k = (uint32_t)e, r = (uint32_t)(e >> 32);
+  const uint32_t k = (uint32_t)e, r = (uint32_t)(e >> 32) & 0x7FFFFFFFu;
   if (k >= nkeys) return;
   if (r == 0) cnt[k] = 0u;
-  idx[offs[k] + r] = (uint32_t)i;
+  idx[offs[k] + r] = (uint32_t)i | (uint32_t)(e >> 63) << 31;
 }
 
 // Exclusive scan of n words by one workgroup, out[n] the total, for the
@@ -368,8 +388,7 @@ constexpr int DEP_U = 4;  // cells per thread and round in k_dep_tile
 // weights are formed here from its position, as the key was (its corner
 // index relative to this tile's first node is -1 for a cell of a lower
 // neighbour's last plane).
-__global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ pos,
-                                                  const int32_t *__restrict__ phase_s, int weight_mode,
+__global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ pos, int weight_mode,
                                                   const uint32_t *__restrict__ idx,
                                                   const uint32_t *__restrict__ slots, DepGrid g, int64_t ls1,
                                                   int64_t ls2, double *__restrict__ prod) {
@@ -405,12 +424,9 @@ __global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ 
 #pragma unroll
     for (int u = 0; u < DEP_U; ++u) c[u] = cn[u];
     double4 P[DEP_U];
-    int ph[DEP_U];
 #pragma unroll
-    for (int u = 0; u < DEP_U; ++u) {
-      P[u] = c[u] != UINT32_MAX ? pos[c[u]] : make_double4(0.0, 0.0, 0.0, 0.0);
-      ph[u] = c[u] != UINT32_MAX && weight_mode != 1 ? phase_s[c[u]] : 1;
-    }
+    for (int u = 0; u < DEP_U; ++u)
+      P[u] = c[u] != UINT32_MAX ? pos[c[u] & 0x7FFFFFFFu] : make_double4(0.0, 0.0, 0.0, 0.0);
     cells_of(v0 + DEP_U * DEP_T, cn);
 #pragma unroll
     for (int u = 0; u < DEP_U; ++u) {
@@ -425,7 +441,8 @@ __global__ __launch_bounds__(DEP_T) void k_dep_tile(const double4 *__restrict__ 
         (void)dep_axis(g, a, pc[a], ci, w0, d[a]);  // a keyed cell reaches the grid
         l[a] = ci - T[a];
       }
-      dep_cell(make_double4(d[0], d[1], d[2], dep_wt(P[u], ph[u], weight_mode)), l, vhi, acc);
+      const int ph = (c[u] >> 31) ? 0 : 1;  // G0 or not, as the key pass found it
+      dep_cell(make_double4(d[0], d[1], d[2], dep_wt(P[u], ph, weight_mode)), l, vhi, acc);
     }
   }
   __syncthreads();
@@ -543,7 +560,7 @@ tm_status tm_run_deposit(tm_ctx *c, const tm_grid *grid, int weight_mode, double
     k_dep_place<<<tm_blocks(n, 256), 256, 0, c->stream>>>(c->d_dkr, n, (uint32_t)nkeys, offs, cnt, c->d_didx);
   k_dep_order<<<tm_blocks(ntiles, 256), 256, 0, c->stream>>>(offs, g, (uint32_t)ntiles, ctr, slots);
   // with no cells every offset is 0 and no cell is read
-  k_dep_tile<<<(unsigned)ntiles, DEP_T, 0, c->stream>>>(c->d_pos, c->d_phase_s, weight_mode, c->d_didx, slots, g,
+  k_dep_tile<<<(unsigned)ntiles, DEP_T, 0, c->stream>>>(c->d_pos, weight_mode, c->d_didx, slots, g,
                                                         grid->localsize[1], grid->localsize[2], prod);
   TM_HIP(c, hipGetLastError());
   c->dep_zero = nkeys + 1;
