@@ -163,6 +163,112 @@ __global__ __launch_bounds__(BD) void k_cg_dir(Op A, const double *__restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// The same, marching along i: a workgroup owns an MJ x MK block of (j, k)
+// columns over a run of planes [i0, i1).  Per plane it forms p_new once per
+// node (and once per node of the block's edge ring, which the neighbouring
+// blocks also form) into LDS; the i-1 and i+1 values of its own column stay
+// in registers.  r and p_old are read about once instead of once per
+// neighbour through the caches.  Each node's p_new and Ap are the values
+// k_cg_dir forms (the same expressions, the neighbours added in the same
+// order); only the partial sums of p_new . Ap group differently.
+constexpr int MJ = 4, MK = 64, MT = MJ * MK;
+constexpr int MHALO = 2 * MJ + 2 * MK;  // the edge ring without its corners
+
+__global__ __launch_bounds__(MT) void k_cg_dir_march(Op A, const double *__restrict__ r,
+                                                     const double *__restrict__ pold, double *__restrict__ pnew,
+                                                     double *__restrict__ Ap, int nseg,
+                                                     const CGState *__restrict__ st, double *__restrict__ part) {
+  __shared__ double pl[2][MJ + 2][MK + 2];
+  __shared__ double sh[MT / 64];
+  if (st->done) return;
+  const double beta = st->beta;
+  const int64_t sx = A.ly * A.lz, sy = A.lz;
+  const int ntk = (int)((A.lz + MK - 1) / MK), ntj = (int)((A.ly + MJ - 1) / MJ);
+  int b = blockIdx.x;
+  const int kt = b % ntk;
+  b /= ntk;
+  const int jt = b % ntj, seg = b / ntj;
+  const int64_t i0 = A.lx * seg / nseg, i1 = A.lx * (seg + 1) / nseg;
+  const int tk = (int)threadIdx.x % MK, tj = (int)threadIdx.x / MK;
+  const int64_t j = (int64_t)jt * MJ + tj, k = (int64_t)kt * MK + tk;
+  const bool own = j < A.ly && k < A.lz;
+  // this thread's place on the edge ring, if any: (a, c) in the (MJ+2) x (MK+2) frame
+  int ha = -1, hc = -1;
+  {
+    const int t = (int)threadIdx.x;
+    if (t < MK) {
+      ha = 0;
+      hc = 1 + t;
+    } else if (t < 2 * MK) {
+      ha = MJ + 1;
+      hc = 1 + t - MK;
+    } else if (t < 2 * MK + MJ) {
+      ha = 1 + t - 2 * MK;
+      hc = 0;
+    } else if (t < MHALO) {
+      ha = 1 + t - 2 * MK - MJ;
+      hc = MK + 1;
+    }
+  }
+  const int64_t hj = (int64_t)jt * MJ + ha - 1, hk = (int64_t)kt * MK + hc - 1;
+  const bool halo = ha >= 0 && hj >= 0 && hj < A.ly && hk >= 0 && hk < A.lz;
+  // plane i into LDS buffer q; returns this column's value
+  auto plane = [&](int64_t i, int q) -> double {
+    double v = 0.0;
+    if (own) {
+      const int64_t node = i * sx + j * sy + k;
+      v = r[node] + beta * pold[node];
+      pl[q][tj + 1][tk + 1] = v;
+    }
+    if (halo) {
+      const int64_t node = i * sx + hj * sy + hk;
+      pl[q][ha][hc] = r[node] + beta * pold[node];
+    }
+    return v;
+  };
+  double acc = 0.0;
+  double pprev = 0.0, pcur = 0.0, pnext = 0.0;
+  if (i0 < i1) {
+    if (i0 > 0 && own) {
+      const int64_t node = (i0 - 1) * sx + j * sy + k;
+      pprev = r[node] + beta * pold[node];
+    }
+    pcur = plane(i0, (int)(i0 & 1));
+  }
+  for (int64_t i = i0; i < i1; ++i) {
+    __syncthreads();  // plane i is in its buffer; the reads of plane i - 1 are done
+    const int q = (int)(i & 1);
+    if (i + 1 < A.lx) pnext = plane(i + 1, q ^ 1);
+    if (own) {
+      double nb = 0.0;
+      if (i > 0) nb += pprev;
+      if (i < A.lx - 1) nb += pnext;
+      if (j > 0) nb += pl[q][tj][tk + 1];
+      if (j < A.ly - 1) nb += pl[q][tj + 2][tk + 1];
+      if (k > 0) nb += pl[q][tj + 1][tk];
+      if (k < A.lz - 1) nb += pl[q][tj + 1][tk + 2];
+      const double ap = A.diag * pcur - A.z * nb;
+      const int64_t node = i * sx + j * sy + k;
+      pnew[node] = pcur;
+      Ap[node] = ap;
+      acc += pcur * ap;
+    }
+    pprev = pcur;
+    pcur = pnext;
+  }
+  __syncthreads();
+  double v = acc;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < MT / 64; ++w) s += sh[w];
+    part[blockIdx.x] = s;
+  }
+}
+
 __global__ void k_cg_alpha(const double *__restrict__ part, int nblk, CGState *st) {
   __shared__ double sh[BD / 64];
   if (st->done) return;
@@ -245,6 +351,12 @@ static tm_status run_diffusion(tm_ctx *c, double *work, const tm_grid *g, const 
   const int64_t nblk0 = std::min<int64_t>(MAXBLK, (total + BD - 1) / BD);
   const int64_t chunk = (((total + nblk0 - 1) / nblk0) + BD - 1) / BD * BD;
   const int nblk = (int)((total + chunk - 1) / chunk);
+  // the marching direction kernel: (j, k) blocks times runs of planes, about
+  // 2,048 workgroups, at most MAXBLK (else the one-lane-per-node kernel)
+  const int64_t ncol = ((ly + MJ - 1) / MJ) * ((lz + MK - 1) / MK);
+  const int nseg = (int)std::max<int64_t>(1, std::min<int64_t>(lx, std::max<int64_t>(1, 2048 / ncol)));
+  const bool march = ncol * nseg <= MAXBLK;
+  const int nblk_dir = march ? (int)(ncol * nseg) : nblk;
   Op A;
   A.lx = lx;
   A.ly = ly;
@@ -267,8 +379,9 @@ static tm_status run_diffusion(tm_ctx *c, double *work, const tm_grid *g, const 
     for (;;) {
       const int32_t batch = std::min<int32_t>(8, d->maxiter - launched);
       for (int32_t q = 0; q < batch; ++q) {
-        k_cg_dir<<<nblk, BD, 0, c->stream>>>(A, r, pold, pnew, Ap, chunk, st, part);
-        k_cg_alpha<<<1, BD, 0, c->stream>>>(part, nblk, st);
+        if (march) k_cg_dir_march<<<nblk_dir, MT, 0, c->stream>>>(A, r, pold, pnew, Ap, nseg, st, part);
+        else k_cg_dir<<<nblk, BD, 0, c->stream>>>(A, r, pold, pnew, Ap, chunk, st, part);
+        k_cg_alpha<<<1, BD, 0, c->stream>>>(part, nblk_dir, st);
         k_cg_update<<<nblk, BD, 0, c->stream>>>(total, field, r, pnew, Ap, chunk, st, part);
         k_cg_beta<<<1, BD, 0, c->stream>>>(part, nblk, d->maxiter, st);
         std::swap(pold, pnew);
