@@ -209,3 +209,24 @@ def test_step_compute_with_no_cells(oracle_mod, tmp_path):
     assert np.all(got["prod"] == 0)
     for k in range(len(fields)):
         assert np.array_equal(got["grad"][k], O.fieldgradient(grid["localsize"], grid["resolution"], fields[k]))
+
+
+def test_step_grid_copies_span_several_chunks(oracle_mod, tmp_path):
+    """A 168^3 grid (38 MB per field, 114 MB per gradient): the layer's copies
+    from and to the caller's arrays run through its 32 MB page-locked chunks
+    several times each; production, the interpolated values and the field
+    gradient still equal the oracle's."""
+    O = oracle_mod
+    col = gen.config(0, scale=0.25)
+    col["grid"] = gen.bbox_grid(col, 168)
+    got, fields, grads = _run(tmp_path, col, "fp64")
+    grid = col["grid"]
+    oc = O.Colony.from_dict(col)
+    ref = O.cic_deposit(oc, grid, 0)
+    p0 = got["prod"][0]
+    assert np.array_equal(p0 != 0, ref != 0)
+    assert np.all(np.abs(p0 - ref) <= 1e-12 * np.abs(ref))
+    assert np.all(got["prod"][1:] == 0)
+    for k in range(len(fields)):
+        assert np.array_equal(got["ced"][k], O.cic_interp(oc, grid, fields[k], grads[k]))
+        assert np.array_equal(got["grad"][k], O.fieldgradient(grid["localsize"], grid["resolution"], fields[k]))
