@@ -1,10 +1,11 @@
 // Device-wide exclusive scan and stable LSD radix sort of (u64 key, i32 value)
 // pairs, 8-bit digits, for the Morton ordering of cells.
 //
-// Each pass: histogram per 4096-key tile -> exclusive scan of the digit-major
-// histogram -> stable scatter.  Stable ranks inside a tile come from 64-lane
-// ballots: the lanes holding the same digit are found with eight ballots,
-// the leader of each group advances a per-wave digit counter in LDS.
+// Each pass: histogram per 4096-key tile (LDS atomics) -> exclusive scan of
+// the digit-major histogram -> stable scatter.  Stable ranks inside a tile
+// come from 64-lane ballots: the lanes holding the same digit are found with
+// eight ballots, the leader of each group advances a per-wave digit counter
+// in LDS.
 #include "tm_internal.h"
 
 namespace {
@@ -43,13 +44,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
     const int64_t i = base + it * 64 + lane;
     dg[it] = i < n ? (uint32_t)((keys[i] >> shift) & 255u) : 0u;
   }
+  // counts only (the ranks are the scatter's), so one LDS atomic per key
+  // into this wave's histogram
 #pragma unroll
-  for (int it = 0; it < RS_ITEMS; ++it) {
-    const bool valid = base + it * 64 + lane < n;
-    const uint32_t d = dg[it];
-    unsigned long long peers = tm_match8(d, valid);
-    if (valid && (peers & tm_lanemask_lt()) == 0) h[w][d] += (uint32_t)__popcll(peers);
-  }
+  for (int it = 0; it < RS_ITEMS; ++it)
+    if (base + it * 64 + lane < n) atomicAdd(&h[w][dg[it]], 1u);
   __syncthreads();
   uint32_t s = 0;
 #pragma unroll
