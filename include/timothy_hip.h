@@ -52,7 +52,8 @@ typedef enum tm_status {
   TM_ERR_INVALID = 1,     /* bad argument or size */
   TM_ERR_HIP = 2,         /* HIP runtime error */
   TM_ERR_NOMEM = 3,       /* device or host allocation failed */
-  TM_ERR_DEGENERATE = 4,  /* two cells share location codes down to bit 1 */
+  TM_ERR_DEGENERATE = 4,  /* two cells share location codes down to bit 1, or more than 64 cells
+                            lie within 2^-16 of their colony's extent on every axis */
   TM_ERR_STATE = 5,       /* call out of order (e.g. tm_sph before tm_neighbors) */
   TM_ERR_UNSUPPORTED = 6  /* e.g. more than one MPI rank in the compat layer */
 } tm_status;

@@ -233,22 +233,29 @@ __global__ __launch_bounds__(BT) void k_gather(const int32_t *__restrict__ perm,
   phase_s[i] = (int32_t)X.z;
 }
 
-// Cells whose 64-bit key prefixes tie are put in full 87-bit key order; the
-// first thread of each run sorts it (runs are a few cells at most).
-__global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key, int64_t n, int32_t *perm, uint4 *code,
+// The sort orders the keys by their bits from SORT_LO_BIT up.  Cells whose
+// keys tie there are put in full 87-bit key order here (and their keys with
+// them); the first thread of each run sorts it (runs are a cell or two at
+// most: a tie needs two cells within 2^-16 of the colony's extent on every
+// axis, a colony id in the key's top bits leaving fewer Morton bits).
+constexpr int SORT_LO_BIT = 16;
+
+__global__ __launch_bounds__(BT) void k_fixties(uint64_t *key, int64_t n, int32_t *perm, uint4 *code,
                                                 double4 *pos, float4 *posf, float *young_s, int32_t *ctype_s,
                                                 int32_t *phase_s, unsigned int *flags) {
   int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
   if (i >= n - 1) return;
-  if (key[i + 1] != key[i]) return;
-  if (i > 0 && key[i - 1] == key[i]) return;
+  const uint64_t ki = key[i] >> SORT_LO_BIT;
+  if (key[i + 1] >> SORT_LO_BIT != ki) return;
+  if (i > 0 && key[i - 1] >> SORT_LO_BIT == ki) return;
   int64_t e = i + 1;
-  while (e < n && key[e] == key[i] && e - i <= 64) ++e;
-  if (e - i > 64) {  // > 64 cells within 2^8 codes on every axis: treated as degenerate
+  while (e < n && key[e] >> SORT_LO_BIT == ki && e - i <= 64) ++e;
+  if (e - i > 64) {  // > 64 cells that close together: treated as degenerate
     atomicOr(flags, (unsigned)TM_FLAG_DEGENERATE);
     return;
   }
   for (int64_t a = i + 1; a < e; ++a) {
+    const uint64_t ka = key[a];
     uint4 c = code[a];
     int32_t pm = perm[a];
     double4 ps = pos[a];
@@ -257,6 +264,7 @@ __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key
     int32_t ct = ctype_s[a], ph = phase_s[a];
     int64_t b = a - 1;
     while (b >= i && morton_less(c, code[b])) {
+      key[b + 1] = key[b];
       code[b + 1] = code[b];
       perm[b + 1] = perm[b];
       pos[b + 1] = pos[b];
@@ -266,6 +274,7 @@ __global__ __launch_bounds__(BT) void k_fixties(const uint64_t *__restrict__ key
       phase_s[b + 1] = phase_s[b];
       --b;
     }
+    key[b + 1] = ka;
     code[b + 1] = c;
     perm[b + 1] = pm;
     pos[b + 1] = ps;
@@ -472,7 +481,7 @@ tm_status tm_run_build(tm_ctx *c) {
                                            c->ncol, cb, c->ntypes, c->d_aff, c->d_key, c->d_perm, c->d_pos_o,
                                            c->d_aux_o);
   TM_HIP(c, hipGetLastError());
-  TM_TRY(tm_sort_pairs(c, c->d_key, c->d_perm, n));
+  TM_TRY(tm_sort_pairs(c, c->d_key, c->d_perm, n, SORT_LO_BIT));
   k_gather<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_perm, n, c->d_pos_o, c->d_aux_o, c->d_aff, c->d_code, c->d_pos,
                                              c->d_posf, c->d_young_s, c->d_ctype_s, c->d_phase_s);
   k_fixties<<<tm_blocks(n, BT), BT, 0, st>>>(c->d_key, n, c->d_perm, c->d_code, c->d_pos, c->d_posf,

@@ -245,7 +245,7 @@ tm_status tm_ensure(tm_ctx *c, T **p, int64_t &cap, int64_t need, const char *wh
 }
 
 // stage entry points implemented per file
-tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n);
+tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n, int lo_bit);
 tm_status tm_scan_u32(tm_ctx *c, const uint32_t *in, uint32_t *out, int64_t n);
 tm_status tm_run_build(tm_ctx *c);
 tm_status tm_run_neighbors(tm_ctx *c, bool check);

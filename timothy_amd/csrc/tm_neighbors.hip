@@ -469,7 +469,8 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     TM_HIP(c, hipStreamSynchronize(c->stream));
     if (c->h_flags[0] & TM_FLAG_DEGENERATE)
       return tm_fail(c, TM_ERR_DEGENERATE,
-                     "two cells share location codes down to bit 1 (duplicate or near-duplicate positions)");
+                     "two cells share location codes down to bit 1, or more than 64 lie within 2^-16 of their colony's "
+                     "extent on every axis (duplicate or near-duplicate positions)");
     if (c->h_flags[0] & TM_FLAG_ROOT) return tm_fail(c, TM_ERR_STATE, "colony root not found in the radix tree");
     if (c->h_flags[0] & TM_FLAG_WALK) return tm_fail(c, TM_ERR_STATE, "malformed tree: walk left its node range");
     if (c->h_flags[0] & TM_FLAG_CTYPE) return tm_fail(c, TM_ERR_INVALID, "ctype out of range");

@@ -297,15 +297,17 @@ tm_status tm_scan_u32(tm_ctx *c, const uint32_t *in, uint32_t *out, int64_t n) {
   return TM_OK;
 }
 
-// Stable sort of (keys, vals) by all 64 key bits; the result ends in the
-// input arrays (eight passes ping-pong through c->d_key_alt / d_val_alt).
-tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n) {
+// Stable sort of (keys, vals) by key bits [lo_bit, 64) (lo_bit a multiple
+// of 16, so the passes are even in number); the result ends in the input
+// arrays (the passes ping-pong through c->d_key_alt / d_val_alt).
+tm_status tm_sort_pairs(tm_ctx *c, uint64_t *keys, int32_t *vals, int64_t n, int lo_bit) {
   if (n <= 1) return TM_OK;
+  if (lo_bit < 0 || lo_bit >= 64 || lo_bit % 16) return tm_fail(c, TM_ERR_INVALID, "sort: lo_bit");
   int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
   TM_TRY(tm_ensure(c, &c->d_hist, c->cap_hist, (int64_t)256 * nblocks + 256, "radix histogram"));
   uint64_t *k0 = keys, *k1 = c->d_key_alt;
   int32_t *v0 = vals, *v1 = c->d_val_alt;
-  for (int shift = 0; shift < 64; shift += 8) {
+  for (int shift = lo_bit; shift < 64; shift += 8) {
     k_rs_hist<<<nblocks, RS_THREADS, 0, c->stream>>>(k0, n, shift, c->d_hist, nblocks);
     TM_HIP(c, hipGetLastError());
     k_rs_rowscan<<<256, RS_THREADS, 0, c->stream>>>(c->d_hist, nblocks);
