@@ -114,7 +114,7 @@ tm_status tm_set_celltypes(tm_ctx *ctx, const tm_celltype *types, int32_t ntypes
  * at most 1279 records per tile when at most one tile in twenty needs more,
  * so four pair-kernel workgroups fit per CU; default 1), "walk_window" slots
  * of the pre-ordered tree a 256-cell walk workgroup stages in LDS around its
- * cells' leaves (0, 512, 1024, 2048; default 1024), "walk_f32" (1 = the walk
+ * cells' leaves (0, 512, 1024, 1280, 2048; default 1280), "walk_f32" (1 = the walk
  * screens candidates with FP32 positions relative to the colony's corner and
  * a proved error bound, reading FP64 positions only where that bound cannot
  * decide; results are identical; default 1), "tile_window" (1 = the tile

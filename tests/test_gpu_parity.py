@@ -105,7 +105,7 @@ def test_launch_shapes_invariant(tm, oracle_mod, ctx):
     ctx.neighbors()
     lists = ctx.neighbor_lists()
     for f32 in (0, 1):
-        for nb, win in ((64, 0), (128, 0), (256, 0), (256, 512), (256, 2048), (256, 1024)):
+        for nb, win in ((64, 0), (128, 0), (256, 0), (256, 512), (256, 1280), (256, 2048), (256, 1024)):
             say(f"nbr_block {nb} walk_window {win} walk_f32 {f32}")
             ctx.set_param("walk_f32", f32)
             ctx.set_param("nbr_block", nb)

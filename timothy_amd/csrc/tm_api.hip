@@ -454,8 +454,8 @@ tm_status tm_set_param(tm_ctx *c, const char *key, int64_t v) {
     if (v != c->tile_width) tm_graph_destroy(c);
     c->tile_width = (int)v;
   } else if (!strcmp(key, "walk_window")) {
-    if (v != 0 && v != 512 && v != 1024 && v != 2048)
-      return tm_fail(c, TM_ERR_INVALID, "walk_window in {0,512,1024,2048}");
+    if (v != 0 && v != 512 && v != 1024 && v != 1280 && v != 2048)
+      return tm_fail(c, TM_ERR_INVALID, "walk_window in {0,512,1024,1280,2048}");
     c->walk_window = (int)v;
   } else {
     return tm_fail(c, TM_ERR_INVALID, std::string("unknown parameter ") + key);
@@ -958,7 +958,7 @@ tm_status tm_autotune(tm_ctx *c, tm_precision p, char *buf, size_t buflen) {
   const int nb0 = c->nbr_block, win0 = c->walk_window;
   int best_nb = nb0, best_win = win0;
   double best_t = 1e30, cur_t = 1e30;
-  const int shapes[][2] = {{64, 0}, {128, 0}, {256, 0}, {256, 512}, {256, 1024}, {256, 2048}};
+  const int shapes[][2] = {{64, 0}, {128, 0}, {256, 0}, {256, 512}, {256, 1024}, {256, 1280}, {256, 2048}};
   for (const auto &sh : shapes) {
     c->nbr_block = sh[0];
     c->walk_window = sh[1];

@@ -69,7 +69,7 @@ struct tm_ctx {
   int sph_unroll_fp64 = 4;   // neighbours fetched per trip in the FP64 pair kernel
   int sph_unroll_mixed = 4;  // the same for the mixed kernel
   int nbr_block = 256;
-  int walk_window = 1024;  // slots of the tree a 256-cell walk workgroup stages in LDS (0, 512, 1024, 2048)
+  int walk_window = 1280;  // slots of the tree a 256-cell walk workgroup stages in LDS (0, 512, 1024, 1280, 2048)
   int walk_f32 = 1;  // the walk screens candidates with FP32 positions and an error bound (results unchanged)
   int sph_lanes = 1;
   int sph_balance = 0;  // FP64 kernel without tiles: rank cells by list length inside each workgroup

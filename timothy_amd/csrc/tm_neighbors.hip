@@ -460,6 +460,7 @@ tm_status tm_run_neighbors(tm_ctx *c, bool check) {
     else if (bs == 64) launch_walk<64, 0>(c);
     else if (bs == 128) launch_walk<128, 0>(c);
     else if (c->walk_window == 2048 && 2 * n - 1 >= 2048) launch_walk<256, 2048>(c);
+    else if (c->walk_window == 1280 && 2 * n - 1 >= 1280) launch_walk<256, 1280>(c);
     else if (c->walk_window == 1024 && 2 * n - 1 >= 1024) launch_walk<256, 1024>(c);
     else if (c->walk_window == 512 && 2 * n - 1 >= 512) launch_walk<256, 512>(c);
     else launch_walk<256, 0>(c);
