@@ -140,9 +140,15 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   constexpr int WPT = WB / 32 / TS;  // bitmap words per thread
   static_assert(WPT * TS * 32 == WB, "bitmap words");
   __shared__ uint32_t keys[HASH];
-  __shared__ uint16_t vals[HASH];
   __shared__ uint32_t list[SORTN];
-  __shared__ uint32_t bins[TS];
+  // list's words are shared by two arrays whose lives do not meet its own:
+  // the ranking's bins (dead before the barrier that ends the neighbour
+  // pass, after which list is first written) and the hash set's positions
+  // (written after every lane has read the sorted set into registers).  The
+  // smaller footprint lets seven workgroups share a CU instead of five.
+  static_assert(TS <= SORTN && HASH * sizeof(uint16_t) <= sizeof(list), "arrays inside list");
+  uint32_t *const bins = list;
+  uint16_t *const vals = reinterpret_cast<uint16_t *>(list);
   __shared__ uint32_t rcnt[TS];
   __shared__ uint16_t order[TS];
   __shared__ uint32_t wsum[TS / 64];
@@ -170,6 +176,18 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   const uint32_t rank = atomicAdd(&bins[b], 1u);
   order[rank] = (uint16_t)t;
   rcnt[rank] = cnt;
+  __syncthreads();
+  // from here on thread t reads rank t's list, so the lists of a wave's
+  // lanes have similar lengths; its first 16 slots stay in registers for the
+  // second pass over it
+  const uint32_t off = order[t];
+  const int64_t src = g * TS + off;
+  const uint32_t rc = rcnt[t];
+  uint32_t p0[8], p1[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) p0[j] = (uint32_t)j < rc ? nbr[tm_slot(src, j, cap)] : EMPTY;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) p1[j] = (uint32_t)(8 + j) < rc ? nbr[tm_slot(src, 8 + j, cap)] : EMPTY;
 
   // Distinct neighbours of the tile.  Most lie within WB / 2 cells of its
   // middle (84% in the 2M-cell Gaussian colony): those are marked in a
@@ -179,11 +197,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   // sorted ones above it.  (Stop counting past TM_TILE_MAX; a lane reads 8
   // list slots per round so their loads overlap.)
   const int64_t wlo = g * TS + TS / 2 - (int64_t)(win / 2);  // may be negative
-  for (uint32_t k0 = 0; k0 < cnt; k0 += 8) {
-    if (*(volatile uint32_t *)&hfull) break;
-    uint32_t qs[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) qs[j] = k0 + j < cnt ? nbr[tm_slot(cell, k0 + j, cap)] : EMPTY;
+  auto insert8 = [&](const uint32_t (&qs)[8]) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const uint32_t q = qs[j];
@@ -219,6 +233,15 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
         h = (h + 1) & (HASH - 1);
       }
     }
+  };
+  if (rc > 0) insert8(p0);
+  if (rc > 8) insert8(p1);
+  for (uint32_t k0 = 16; k0 < rc; k0 += 8) {
+    if (*(volatile uint32_t *)&hfull) break;
+    uint32_t qs[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qs[j] = k0 + j < rc ? nbr[tm_slot(src, k0 + j, cap)] : EMPTY;
+    insert8(qs);
   }
   __syncthreads();
   // the window's count and prefix (thread t holds words WPT t .. WPT t + WPT - 1)
@@ -273,9 +296,20 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   else if (TS * 4 <= SORTN && mo <= 4 * TS) tile_sort<TS, (TS * 4 <= SORTN ? 4 : 1)>(list, mo, t);
   else if (TS * 8 <= SORTN && mo <= 8 * TS) tile_sort<TS, (TS * 8 <= SORTN ? 8 : 1)>(list, mo, t);
   else tile_sort<TS, SORTN / TS>(list, mo, t);
-  // positions: the sorted ones ...
-  for (uint32_t i = t; i < mo; i += TS) {
-    const uint32_t q = list[i];
+  // positions: the sorted ones (read out of list before vals overwrites it) ...
+  constexpr int EV = SORTN / TS;
+  uint32_t qv[EV];
+#pragma unroll
+  for (int e = 0; e < EV; ++e) {
+    const uint32_t i = t + (uint32_t)TS * e;
+    qv[e] = i < mo ? list[i] : EMPTY;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < EV; ++e) {
+    const uint32_t i = t + (uint32_t)TS * e;
+    if (i >= mo) break;
+    const uint32_t q = qv[e];
     const uint32_t pos = i < mb ? i : i + mw;
     uint32_t h = hslot<HBITS>(q);
     while (keys[h] != q) h = (h + 1) & (HASH - 1);
@@ -303,9 +337,6 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   __syncthreads();
 
   // lists by rank: thread t writes rank t's list (rows of a wave coalesced)
-  const uint32_t off = order[t];
-  const int64_t src = g * TS + off;
-  const uint32_t rc = rcnt[t];
   uint32_t pad = rc;  // longest list of this wave of ranks
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) pad = max(pad, (uint32_t)__shfl_xor((int)pad, o, 64));
@@ -313,10 +344,7 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
   const int64_t r = g * TS + t;
   const uint32_t pad8 = (pad + 7) & ~7u;  // <= cap, a multiple of 8
   // one 16-byte chunk of 8 slots per round: its 8 list loads overlap
-  for (uint32_t k0 = 0; k0 < pad8; k0 += 8) {
-    uint32_t qs[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) qs[j] = k0 + j < rc ? nbr[tm_slot(src, k0 + j, cap)] : EMPTY;
+  auto emit8 = [&](uint32_t k0, const uint32_t (&qs)[8]) {
     uint32_t v[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -336,6 +364,14 @@ __global__ __launch_bounds__(TS) void k_tiles(int64_t n, int64_t cap, uint32_t w
     }
     *reinterpret_cast<uint4 *>(nbr16 + tm_nbr16_at(r, k0, cap)) =
         make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
+  };
+  if (pad8 > 0) emit8(0, p0);
+  if (pad8 > 8) emit8(8, p1);
+  for (uint32_t k0 = 16; k0 < pad8; k0 += 8) {
+    uint32_t qs[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qs[j] = k0 + j < rc ? nbr[tm_slot(src, k0 + j, cap)] : EMPTY;
+    emit8(k0, qs);
   }
 }
 
