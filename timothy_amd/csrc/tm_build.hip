@@ -80,10 +80,10 @@ __device__ __forceinline__ int key_delta(const uint4 *__restrict__ code, int64_t
 // (cb top bits the colony id, then the Morton key's leading 64 - cb bits,
 // z before y before x in each triple as in the code comparison above), the
 // codes only where the keys tie: half the bytes per probe of the searches.
-__device__ __forceinline__ int key_delta64(const uint64_t *__restrict__ key, uint64_t ki, const uint4 *__restrict__ code,
-                                           int64_t n, int cb, int64_t i, int64_t j) {
-  if (j < 0 || j >= n) return -1;
-  const uint64_t x = ki ^ key[j];
+// ki and kj are the keys of cells i and j, both in [0, n).
+__device__ __forceinline__ int key_delta64(uint64_t ki, uint64_t kj, const uint4 *__restrict__ code, int64_t n,
+                                           int cb, int64_t i, int64_t j) {
+  const uint64_t x = ki ^ kj;
   if (x) {
     const int p = __clzll((long long)x);
     return p < cb ? TM_CID_BITS - cb + p : TM_CID_BITS + p - cb;
@@ -327,10 +327,24 @@ __global__ __launch_bounds__(BT) void k_karras(const uint4 *__restrict__ code, c
                                                int64_t n, int cb, int2 *__restrict__ range,
                                                int32_t *__restrict__ delta, int32_t *__restrict__ split,
                                                uint32_t *__restrict__ chain) {
-  int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x;
+  // the keys of this workgroup's cells and of BT cells on either side, in
+  // LDS: the searches of nodes spanning up to about BT cells (nearly all of
+  // them) probe only there
+  __shared__ uint64_t kw[3 * BT];
+  const int64_t base = (int64_t)blockIdx.x * BT;
+  for (int u = threadIdx.x; u < 3 * BT; u += BT) {
+    const int64_t jj = base - BT + u;
+    kw[u] = jj >= 0 && jj < n ? key[jj] : 0ull;
+  }
+  __syncthreads();
+  const int64_t i = base + threadIdx.x;
   if (i >= n - 1) return;
-  const uint64_t ki = key[i];
-  auto dl = [&](int64_t j) { return key_delta64(key, ki, code, n, cb, i, j); };
+  const uint64_t ki = kw[BT + threadIdx.x];
+  auto dl = [&](int64_t jj) -> int {
+    if (jj < 0 || jj >= n) return -1;
+    const uint64_t u = (uint64_t)(jj - (base - BT));
+    return key_delta64(ki, u < (uint64_t)(3 * BT) ? kw[u] : key[jj], code, n, cb, i, jj);
+  };
   int d = (dl(i + 1) - dl(i - 1)) >= 0 ? 1 : -1;
   int dmin = dl(i - d);
   int64_t lmax = 2;
